@@ -1,0 +1,475 @@
+// C ABI of libiactrace_amd: argument validation, launch shaping and the
+// extern "C" entry points declared in include/iactrace_amd.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "iact_abi_internal.h"
+
+namespace iact {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_error(hipError_t e, const char* where) {
+  return set_error(IACT_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------- device info
+struct DevInfo {
+  int n_cu = 0;
+  int lds_per_block = 0;
+  char arch[64] = {0};
+};
+static std::mutex g_dev_mu;
+static DevInfo g_dev[64];
+static bool g_dev_ok[64];
+
+int device_info(DevInfo& out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) return set_error(IACT_ERR_UNSUPPORTED, "device ordinal %d", dev);
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  if (!g_dev_ok[dev]) {
+    hipDeviceProp_t p;
+    e = hipGetDeviceProperties(&p, dev);
+    if (e != hipSuccess) return hip_error(e, "hipGetDeviceProperties");
+    g_dev[dev].n_cu = p.multiProcessorCount;
+    int lds = 0;
+    e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return hip_error(e, "hipDeviceGetAttribute");
+    g_dev[dev].lds_per_block = lds;
+    strncpy(g_dev[dev].arch, p.gcnArchName, sizeof(g_dev[dev].arch) - 1);
+    g_dev_ok[dev] = true;
+  }
+  out = g_dev[dev];
+  return IACT_OK;
+}
+
+// ------------------------------------------------------------- validation
+static bool finite3(const float* v) {
+  return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
+}
+
+int validate_facets(const iact_facets& F) {
+  if (F.n_facets < 0 || F.n_samples < 0)
+    return set_error(IACT_ERR_INVALID, "negative facet/sample count (%d, %d)", F.n_facets,
+                     F.n_samples);
+  if (F.n_facets > 0 && F.n_samples == 0)
+    return set_error(IACT_ERR_INVALID, "facets have no samples (resample the mirrors)");
+  if (F.n_facets == 0) return IACT_OK;
+  if (!F.points || !F.normals || !F.delta || !F.weights || !F.positions || !F.rotations ||
+      !F.perturbation_scale)
+    return set_error(IACT_ERR_INVALID, "null facet buffer");
+  if ((long long)F.n_facets * F.n_samples > (1ll << 40))
+    return set_error(IACT_ERR_UNSUPPORTED, "facet sample count too large");
+  return IACT_OK;
+}
+
+int validate_sensor(const iact_sensor& S) {
+  if (S.kind != IACT_SENSOR_SQUARE && S.kind != IACT_SENSOR_HEX)
+    return set_error(IACT_ERR_INVALID, "unknown sensor kind %d", S.kind);
+  if (S.n_pixels <= 0) return set_error(IACT_ERR_INVALID, "sensor has no pixels");
+  if (!finite3(S.position)) return set_error(IACT_ERR_INVALID, "non-finite sensor position");
+  for (int i = 0; i < 9; ++i)
+    if (!isfinite(S.rotation[i])) return set_error(IACT_ERR_INVALID, "non-finite sensor frame");
+  if (S.kind == IACT_SENSOR_SQUARE) {
+    if (S.width <= 0 || S.height <= 0 || (long long)S.width * S.height != S.n_pixels)
+      return set_error(IACT_ERR_INVALID, "square sensor %dx%d does not match %d pixels", S.width,
+                       S.height, S.n_pixels);
+    if (!(S.dx > 0.f) || !(S.dy > 0.f))
+      return set_error(IACT_ERR_INVALID, "square sensor pixel pitch must be > 0");
+  } else {
+    if (!S.lookup_table) return set_error(IACT_ERR_INVALID, "hex sensor without lookup table");
+    if (S.table_q <= 0 || S.table_r <= 0)
+      return set_error(IACT_ERR_INVALID, "hex lookup table shape %dx%d", S.table_q, S.table_r);
+    if (!(S.hex_size > 0.f) || !(S.hex_inradius > 0.f))
+      return set_error(IACT_ERR_INVALID, "hex pixel size must be > 0");
+  }
+  if (S.splat) {
+    if (!(S.sigma > 0.f)) return set_error(IACT_ERR_INVALID, "splat sigma must be > 0");
+    if (S.kernel_size < 0 || S.kernel_size > 4)
+      return set_error(IACT_ERR_UNSUPPORTED, "splat kernel_size %d (0..4)", S.kernel_size);
+  }
+  return IACT_OK;
+}
+
+int validate_scene(const iact_scene* sc) {
+  if (!sc) return set_error(IACT_ERR_INVALID, "null scene");
+  int rc = validate_facets(sc->facets);
+  if (rc) return rc;
+  if (sc->n_surfaces < 0 || sc->n_surfaces > IACT_MAX_SURFACES)
+    return set_error(IACT_ERR_INVALID, "n_surfaces %d (max %d)", sc->n_surfaces,
+                     IACT_MAX_SURFACES);
+  for (int i = 0; i < sc->n_surfaces; ++i) {
+    const iact_surface& s = sc->surfaces[i];
+    if (s.n_aspheric < 0 || s.n_aspheric > IACT_MAX_ASPHERIC)
+      return set_error(IACT_ERR_UNSUPPORTED, "surface %d has %d aspheric terms (max %d)", i,
+                       s.n_aspheric, IACT_MAX_ASPHERIC);
+  }
+  if (sc->n_mirrors < 0) return set_error(IACT_ERR_INVALID, "negative mirror count");
+  if (sc->n_mirrors > 0 && !sc->mirrors) return set_error(IACT_ERR_INVALID, "null mirror table");
+  if (sc->n_mirrors > 0 && sc->n_surfaces == 0)
+    return set_error(IACT_ERR_INVALID, "stage>=1 mirrors but no surfaces");
+  if (sc->n_stages < 0 || sc->n_stages > 8)
+    return set_error(IACT_ERR_UNSUPPORTED, "n_stages %d (max 8)", sc->n_stages);
+  if (sc->n_stages > 0 && sc->n_mirrors == 0)
+    return set_error(IACT_ERR_INVALID, "stages listed without mirrors");
+  for (int i = 0; i < sc->n_stages; ++i)
+    if (sc->stages[i] < 1 || (i && sc->stages[i] <= sc->stages[i - 1]))
+      return set_error(IACT_ERR_INVALID, "stage list must be ascending and >= 1");
+  const iact_obstructions& o = sc->obstructions;
+  const int counts[5] = {o.n_cylinders, o.n_boxes, o.n_spheres, o.n_oriented_boxes,
+                         o.n_triangles};
+  const float* ptrs[5] = {o.cylinders, o.boxes, o.spheres, o.oriented_boxes, o.triangles};
+  for (int i = 0; i < 5; ++i) {
+    if (counts[i] < 0) return set_error(IACT_ERR_INVALID, "negative obstruction count");
+    if (counts[i] > 0 && !ptrs[i]) return set_error(IACT_ERR_INVALID, "null obstruction table");
+  }
+  return validate_sensor(sc->sensor);
+}
+
+// ------------------------------------------------------------- derived data
+DevSensor make_dev_sensor(const iact_sensor& s) {
+  DevSensor S;
+  memset(&S, 0, sizeof(S));
+  S.kind = s.kind;
+  S.splat = s.splat;
+  S.n_pixels = s.n_pixels;
+  S.pos = f3{s.position[0], s.position[1], s.position[2]};
+  const float* R = s.rotation;  // row-major; columns are u1, u2, n
+  S.u1 = f3{R[0], R[3], R[6]};
+  S.u2 = f3{R[1], R[4], R[7]};
+  S.nrm = f3{R[2], R[5], R[8]};
+  S.ndotp = S.nrm.x * S.pos.x + S.nrm.y * S.pos.y + S.nrm.z * S.pos.z;
+  S.width = s.width;
+  S.height = s.height;
+  S.x0 = s.x0;
+  S.y0 = s.y0;
+  S.dx = s.dx;
+  S.dy = s.dy;
+  S.inv_dx = s.dx > 0.f ? 1.f / s.dx : 0.f;
+  S.inv_dy = s.dy > 0.f ? 1.f / s.dy : 0.f;
+  S.wf = (float)s.width;
+  S.hf = (float)s.height;
+  S.edge_width = s.edge_width;
+  S.size = s.hex_size;
+  S.inv_size = s.hex_size > 0.f ? 1.f / s.hex_size : 0.f;
+  S.size_s3 = (float)((double)s.hex_size * 1.7320508075688772);
+  S.size_15 = (float)((double)s.hex_size * 1.5);
+  S.inv_inradius = s.hex_inradius > 0.f ? 1.f / s.hex_inradius : 0.f;
+  S.gcos = cosf(-s.grid_rotation);
+  S.gsin = sinf(-s.grid_rotation);
+  S.gox = s.grid_offset[0];
+  S.goy = s.grid_offset[1];
+  S.edge_thr = s.edge_threshold;
+  S.q_min = s.q_min;
+  S.r_min = s.r_min;
+  S.tq = s.table_q;
+  S.tr = s.table_r;
+  S.qmin_f = (float)s.q_min;
+  S.rmin_f = (float)s.r_min;
+  S.tq_f = (float)s.table_q;
+  S.tr_f = (float)s.table_r;
+  S.table = s.lookup_table;
+  S.neg_half_inv_sigma2 = s.sigma > 0.f ? -0.5f / (s.sigma * s.sigma) : 0.f;
+  S.ks = s.kernel_size;
+  return S;
+}
+
+void fill_trace_args(TraceArgs& A, const iact_scene& sc, const double* sources,
+                     const float* values, int ns) {
+  memset(&A, 0, sizeof(A));
+  const iact_facets& F = sc.facets;
+  A.nf = F.n_facets;
+  A.M = F.n_samples;
+  A.cpf = (F.n_samples + 63) / 64;
+  A.points = F.points;
+  A.normals = F.normals;
+  A.delta = F.delta;
+  A.weights = F.weights;
+  A.positions = F.positions;
+  A.rotations = F.rotations;
+  A.pscale = F.perturbation_scale;
+  A.sources = sources;
+  A.values = values;
+  A.ns = ns;
+  A.n_mirrors = sc.n_mirrors;
+  A.n_stages = sc.n_stages;
+  for (int i = 0; i < sc.n_stages; ++i) A.stages[i] = sc.stages[i];
+  A.mirrors = sc.mirrors;
+  A.n_surfaces = sc.n_surfaces;
+  for (int i = 0; i < sc.n_surfaces; ++i) A.surfaces[i] = sc.surfaces[i];
+  A.ob = sc.obstructions;
+  A.S = make_dev_sensor(sc.sensor);
+  A.max_rounds = 16;
+}
+
+// Resolve the launch shape of an image/debug trace.  Returns bytes of LDS.
+int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, TraceArgs& A,
+                LaunchShape& out, const DevInfo& dev) {
+  const int threads = (L && L->block_threads) ? L->block_threads : 256;
+  if (threads % 64 || threads < 64 || threads > IACT_MAX_BLOCK)
+    return set_error(IACT_ERR_INVALID, "block_threads %d must be a multiple of 64 in [64,%d]",
+                     threads, IACT_MAX_BLOCK);
+  const int wpb = threads / 64;
+  const int bpc = (L && L->blocks_per_cu) ? L->blocks_per_cu : 4;
+  if (bpc < 1 || bpc > 64) return set_error(IACT_ERR_INVALID, "blocks_per_cu %d", bpc);
+  int ts = (L && L->source_tile) ? L->source_tile : 64;
+  if (ts < 1) return set_error(IACT_ERR_INVALID, "source_tile %d", ts);
+  if (ts > ns) ts = ns > 0 ? ns : 1;
+  A.source_tile = ts;
+  A.n_stiles = ns > 0 ? (ns + ts - 1) / ts : 0;
+  A.det = (L && L->deterministic) ? 1 : 0;
+  const int P = sc.sensor.n_pixels;
+  const size_t tab = sc.sensor.kind == IACT_SENSOR_HEX
+                         ? (size_t)sc.sensor.table_q * sc.sensor.table_r * sizeof(int32_t)
+                         : 0;
+  size_t budget = (size_t)(dev.lds_per_block > 0 ? dev.lds_per_block : 65536);
+  if (budget > 65536) budget = 65536;
+  if (tab > budget / 2)
+    return set_error(IACT_ERR_UNSUPPORTED, "hex lookup table (%zu bytes) exceeds LDS budget", tab);
+  A.lds_table = tab > 0;
+  size_t img = 0;
+  if (mode == IACT_MODE_IMAGE || mode == IACT_MODE_RESPONSE) {
+    const size_t per = (size_t)P * sizeof(float);
+    const size_t copies = (mode == IACT_MODE_IMAGE && A.det) ? wpb : 1;
+    if (per * copies + tab <= budget) {
+      img = per * copies;
+      A.lds_image = 1;
+    } else if (A.det) {
+      return set_error(IACT_ERR_UNSUPPORTED,
+                       "deterministic mode needs %zu bytes of LDS per block (have %zu)",
+                       per * copies + tab, budget);
+    }
+  }
+  out.threads = threads;
+  out.lds = img + tab;
+  const long long nchunks = (long long)A.nf * A.cpf;
+  A.n_items = nchunks * A.n_stiles;
+  long long grid;
+  if (mode == IACT_MODE_RESPONSE)
+    grid = ns;
+  else
+    grid = (A.n_items + wpb - 1) / wpb;
+  const long long cap = (long long)dev.n_cu * bpc;
+  if (grid > cap) grid = cap;
+  if (grid < 1) grid = 1;
+  out.grid = (int)grid;
+  return IACT_OK;
+}
+
+size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
+                           const TraceArgs& A) {
+  if (mode == IACT_MODE_RESPONSE)
+    return (size_t)7 * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
+  if (mode == IACT_MODE_IMAGE && A.det) return (size_t)shp.grid * sc.sensor.n_pixels * sizeof(float);
+  (void)ns;
+  return 0;
+}
+
+}  // namespace iact
+
+using namespace iact;
+
+extern "C" {
+
+int32_t iact_abi_version(void) { return IACT_ABI_VERSION; }
+
+int64_t iact_struct_size(int32_t which) {
+  switch (which) {
+    case 0: return (int64_t)sizeof(iact_surface);
+    case 1: return (int64_t)sizeof(iact_facets);
+    case 2: return (int64_t)sizeof(iact_obstructions);
+    case 3: return (int64_t)sizeof(iact_sensor);
+    case 4: return (int64_t)sizeof(iact_scene);
+    case 5: return (int64_t)sizeof(iact_launch);
+    case 6: return (int64_t)sizeof(iact_grads);
+    case 7: return (int64_t)sizeof(iact_sample_desc);
+    case 8: return (int64_t)sizeof(iact_batch);
+    default: return -1;
+  }
+}
+
+int32_t iact_sensor_accumulate(const iact_sensor* sensor, const float* x, const float* y,
+                               const float* v, int64_t n, float* out, int32_t* pixel,
+                               void* stream) {
+  if (!sensor) return set_error(IACT_ERR_INVALID, "null sensor");
+  int rc = validate_sensor(*sensor);
+  if (rc) return rc;
+  if (n < 0) return set_error(IACT_ERR_INVALID, "negative point count");
+  if (!out) return set_error(IACT_ERR_INVALID, "null output");
+  if (n > 0 && (!x || !y || !v)) return set_error(IACT_ERR_INVALID, "null point buffers");
+  if (sensor->kind == IACT_SENSOR_HEX &&
+      (size_t)sensor->table_q * sensor->table_r * sizeof(int32_t) > 32768)
+    return set_error(IACT_ERR_UNSUPPORTED, "hex lookup table exceeds 32 KiB of LDS");
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = launch_fill(out, sensor->n_pixels, 0.f, st);
+  if (e != hipSuccess) return hip_error(e, "clear image");
+  if (n == 0) return IACT_OK;
+  const DevSensor S = make_dev_sensor(*sensor);
+  e = launch_accumulate(S, x, y, v, n, out, pixel, st);
+  return e == hipSuccess ? IACT_OK : hip_error(e, "k_accumulate");
+}
+
+const char* iact_last_error(void) { return g_last_error.c_str(); }
+
+int32_t iact_device_info(int32_t device, int32_t* n_cu, int32_t* lds_per_block, char* arch,
+                         size_t arch_len) {
+  if (device < 0) return set_error(IACT_ERR_INVALID, "device %d", device);
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+  if (cur != device) {
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_error(e, "hipSetDevice");
+  }
+  DevInfo d;
+  const int rc = device_info(d);
+  if (cur != device) (void)hipSetDevice(cur);
+  if (rc) return rc;
+  if (n_cu) *n_cu = d.n_cu;
+  if (lds_per_block) *lds_per_block = d.lds_per_block;
+  if (arch && arch_len) {
+    strncpy(arch, d.arch, arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  return IACT_OK;
+}
+
+int32_t iact_sample_facets(const iact_sample_desc* D, float* points, float* normals, float* delta,
+                           float* weights, void* stream) {
+  if (!D) return set_error(IACT_ERR_INVALID, "null sample descriptor");
+  if (D->n_facets < 0 || D->n_samples < 0)
+    return set_error(IACT_ERR_INVALID, "negative facet/sample count");
+  if (D->n_facets == 0 || D->n_samples == 0) return IACT_OK;
+  if (!points || !normals || !delta || !weights || !D->offsets)
+    return set_error(IACT_ERR_INVALID, "null sampling buffer");
+  if (D->aperture == IACT_APERTURE_DISK) {
+    if (!D->radii) return set_error(IACT_ERR_INVALID, "disk group without radii");
+  } else if (D->aperture == IACT_APERTURE_POLYGON) {
+    if (!D->vertices) return set_error(IACT_ERR_INVALID, "polygon group without vertices");
+    if (D->n_vertices < 3 || D->n_vertices > IACT_MAX_POLY_VERTS)
+      return set_error(IACT_ERR_UNSUPPORTED, "polygon with %d vertices (3..%d)", D->n_vertices,
+                       IACT_MAX_POLY_VERTS);
+  } else {
+    return set_error(IACT_ERR_INVALID, "unknown aperture type %d", D->aperture);
+  }
+  if (D->surface.n_aspheric < 0 || D->surface.n_aspheric > IACT_MAX_ASPHERIC)
+    return set_error(IACT_ERR_UNSUPPORTED, "%d aspheric terms (max %d)", D->surface.n_aspheric,
+                     IACT_MAX_ASPHERIC);
+  const hipError_t e = launch_sample(*D, points, normals, delta, weights, (hipStream_t)stream);
+  return e == hipSuccess ? IACT_OK : hip_error(e, "k_sample_facets");
+}
+
+int32_t iact_facet_world(const iact_facets* F, float* out, void* stream) {
+  if (!F) return set_error(IACT_ERR_INVALID, "null facets");
+  int rc = validate_facets(*F);
+  if (rc) return rc;
+  if (F->n_facets == 0) return IACT_OK;
+  if (!out) return set_error(IACT_ERR_INVALID, "null output");
+  DevSensor S;
+  memset(&S, 0, sizeof(S));
+  const hipError_t e = launch_world(*F, S, out, (hipStream_t)stream);
+  return e == hipSuccess ? IACT_OK : hip_error(e, "k_facet_world");
+}
+
+size_t iact_workspace_size(const iact_scene* scene, int32_t n_sources, int32_t mode,
+                           const iact_launch* launch) {
+  if (validate_scene(scene) || n_sources < 0) return 0;
+  DevInfo dev;
+  if (device_info(dev)) return 0;
+  TraceArgs A;
+  fill_trace_args(A, *scene, nullptr, nullptr, n_sources);
+  LaunchShape shp;
+  if (shape_trace(*scene, n_sources, mode, launch, A, shp, dev)) return 0;
+  return workspace_bytes_for(*scene, n_sources, mode, shp, A);
+}
+
+int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const float* values,
+                       int32_t n_sources, int32_t source_type, int32_t mode,
+                       const iact_launch* launch, void* workspace, size_t workspace_bytes,
+                       float* out, float* debug_xy, int32_t* debug_pixel, void* stream) {
+  int rc = validate_scene(scene);
+  if (rc) return rc;
+  if (n_sources < 0) return set_error(IACT_ERR_INVALID, "negative source count");
+  if (n_sources > 0 && (!sources || !values))
+    return set_error(IACT_ERR_INVALID, "null sources/values");
+  if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL)
+    return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
+  if (mode != IACT_MODE_IMAGE && mode != IACT_MODE_RESPONSE && mode != IACT_MODE_DEBUG)
+    return set_error(IACT_ERR_INVALID, "unknown mode %d", mode);
+  const iact_scene& sc = *scene;
+  const long long P = sc.sensor.n_pixels;
+  const long long nrays = (long long)sc.facets.n_facets * n_sources * sc.facets.n_samples;
+  if (mode == IACT_MODE_DEBUG && nrays > 0 && !debug_xy)
+    return set_error(IACT_ERR_INVALID, "debug mode needs debug_xy");
+  const bool writes = mode == IACT_MODE_IMAGE || (mode == IACT_MODE_RESPONSE && n_sources > 0) ||
+                      (mode == IACT_MODE_DEBUG && nrays > 0);
+  if (!out && writes) return set_error(IACT_ERR_INVALID, "null output");
+  hipStream_t st = (hipStream_t)stream;
+  DevInfo dev;
+  rc = device_info(dev);
+  if (rc) return rc;
+  TraceArgs A;
+  fill_trace_args(A, sc, sources, values, n_sources);
+  LaunchShape shp;
+  rc = shape_trace(sc, n_sources, mode, launch, A, shp, dev);
+  if (rc) return rc;
+  const size_t need = workspace_bytes_for(sc, n_sources, mode, shp, A);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                     workspace_bytes);
+  A.out = out;
+  A.dbg_xy = debug_xy;
+  A.dbg_pix = debug_pixel;
+  hipError_t e = hipSuccess;
+  const bool empty = (sc.facets.n_facets == 0 || n_sources == 0);
+  if (mode == IACT_MODE_IMAGE) {
+    if (!(A.det && A.lds_image && !empty)) {
+      e = launch_fill(out, P, 0.f, st);
+      if (e != hipSuccess) return hip_error(e, "clear image");
+    }
+    if (empty) return IACT_OK;
+    A.partials = (float*)workspace;
+    e = launch_trace(A, shp, source_type, mode, st);
+    if (e != hipSuccess) return hip_error(e, "k_trace");
+    if (A.det && A.lds_image) {
+      e = launch_reduce(A.partials, shp.grid, (int)P, out, st);
+      if (e != hipSuccess) return hip_error(e, "k_reduce_partials");
+    }
+    return IACT_OK;
+  }
+  if (mode == IACT_MODE_RESPONSE) {
+    if (n_sources > 0 && (empty || !A.lds_image)) {
+      e = launch_fill(out, (long long)n_sources * P, 0.f, st);
+      if (e != hipSuccess) return hip_error(e, "clear response matrix");
+    }
+    if (empty) return IACT_OK;
+    float* world = (float*)workspace;
+    e = launch_world(sc.facets, A.S, world, st);
+    if (e != hipSuccess) return hip_error(e, "k_facet_world");
+    A.world = world;
+    e = launch_trace(A, shp, source_type, mode, st);
+    return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_response");
+  }
+  if (empty) return IACT_OK;
+  e = launch_trace(A, shp, source_type, mode, st);
+  return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace(debug)");
+}
+
+}  // extern "C"

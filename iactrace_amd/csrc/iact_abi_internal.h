@@ -1,0 +1,18 @@
+// Helpers shared by the C ABI translation units.
+#pragma once
+
+#include "iact_kernels.h"
+
+namespace iact {
+
+struct DevInfo;
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* where);
+int validate_scene(const iact_scene* sc);
+int validate_facets(const iact_facets& F);
+int validate_sensor(const iact_sensor& S);
+DevSensor make_dev_sensor(const iact_sensor& s);
+void fill_trace_args(TraceArgs& A, const iact_scene& sc, const double* sources,
+                     const float* values, int ns);
+
+}  // namespace iact

@@ -1,0 +1,484 @@
+// Device-side building blocks of the gfx950 ray tracer: vector helpers,
+// Philox RNG, obstruction primitives, conic/asphere root refinement, sensor
+// binning and wave-level pixel aggregation.  Every function operates on one
+// ray held by one lane; tables are read with wave-uniform indices so they
+// come through the scalar cache (or LDS for per-lane gathers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/iactrace_amd.h"
+
+#define IACT_DEV __device__ __forceinline__
+
+namespace iact {
+
+// ---------------------------------------------------------------- vectors
+struct f3 {
+  float x, y, z;
+};
+IACT_DEV f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+IACT_DEV f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+IACT_DEV f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+IACT_DEV f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+IACT_DEV f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+IACT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+IACT_DEV f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// 3x3 row-major matrix
+struct m3 {
+  float m[9];
+};
+IACT_DEV f3 mul(const m3& R, f3 v) {  // R v
+  return f3{R.m[0] * v.x + R.m[1] * v.y + R.m[2] * v.z, R.m[3] * v.x + R.m[4] * v.y + R.m[5] * v.z,
+            R.m[6] * v.x + R.m[7] * v.y + R.m[8] * v.z};
+}
+IACT_DEV f3 mulT(const m3& R, f3 v) {  // R^T v
+  return f3{R.m[0] * v.x + R.m[3] * v.y + R.m[6] * v.z, R.m[1] * v.x + R.m[4] * v.y + R.m[7] * v.z,
+            R.m[2] * v.x + R.m[5] * v.y + R.m[8] * v.z};
+}
+
+// Fast hardware approximations (about 1 ulp).  The reference divides and
+// takes square roots with IEEE rounding; these differ by at most a few ulp,
+// which only moves rays that lie within the reported edge band.
+IACT_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+IACT_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+IACT_DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Euler (tip, tilt, rotation) in degrees -> R = Rz Ry Rx (transforms.py:72-107)
+IACT_DEV m3 euler_to_matrix(float tip, float tilt, float rot) {
+  const float k = 0.017453292519943295f;
+  float sx, cx, sy, cy, sz, cz;
+  sincosf(tip * k, &sx, &cx);
+  sincosf(tilt * k, &sy, &cy);
+  sincosf(rot * k, &sz, &cz);
+  m3 R;
+  R.m[0] = cz * cy;
+  R.m[1] = cz * sy * sx - sz * cx;
+  R.m[2] = cz * sy * cx + sz * sx;
+  R.m[3] = sz * cy;
+  R.m[4] = sz * sy * sx + cz * cx;
+  R.m[5] = sz * sy * cx - cz * sx;
+  R.m[6] = -sy;
+  R.m[7] = cy * sx;
+  R.m[8] = cy * cx;
+  return R;
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3"
+// (SC'11): 10 rounds of two 32x32->64 multiplies, Weyl key schedule.
+struct u4 {
+  uint32_t x, y, z, w;
+};
+IACT_DEV u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+IACT_DEV float u01(uint32_t u) { return (float)(u >> 8) * 5.9604644775390625e-08f; }
+IACT_DEV float u01_open(uint32_t u) { return (float)((u >> 8) + 1u) * 5.9604644775390625e-08f; }
+
+// ---------------------------------------------------------------- surfaces
+// sag_raw and its slope (surfaces.py:25-58) in float or double.
+template <typename T>
+struct SurfT {
+  T c, k, K;  // K = (1+k) c^2
+  int na;
+  T a[IACT_MAX_ASPHERIC];
+};
+
+template <typename T>
+IACT_DEV SurfT<T> load_surface(const iact_surface& s) {
+  SurfT<T> o;
+  o.c = (T)s.curvature;
+  o.k = (T)s.conic;
+  o.K = ((T)1 + o.k) * o.c * o.c;
+  o.na = s.n_aspheric;
+#pragma unroll
+  for (int i = 0; i < IACT_MAX_ASPHERIC; ++i) o.a[i] = (T)s.aspheric[i];
+  return o;
+}
+
+template <typename T>
+IACT_DEV T sqrt_t(T x);
+template <>
+IACT_DEV float sqrt_t<float>(float x) { return fsqrt(x); }
+template <>
+IACT_DEV double sqrt_t<double>(double x) { return __builtin_sqrt(x); }
+
+template <typename T>
+IACT_DEV T sag_raw(const SurfT<T>& s, T X, T Y) {
+  const T r2 = X * X + Y * Y;
+  const T den = (T)1 + sqrt_t<T>((T)1 - s.K * r2);
+  T z = r2 * s.c / den;
+  const T r4 = r2 * r2;
+  T p = r4;
+#pragma unroll
+  for (int i = 0; i < IACT_MAX_ASPHERIC; ++i) {
+    if (i < s.na) z += s.a[i] * p;
+    p *= r4;
+  }
+  return z;
+}
+
+// sag and d sag / d r2 at the same point (shares the square root).
+template <typename T>
+IACT_DEV void sag_and_dr2(const SurfT<T>& s, T r2, T& z, T& g) {
+  const T sq = sqrt_t<T>((T)1 - s.K * r2);
+  const T den = (T)1 + sq;
+  const T inv_den = (T)1 / den;
+  z = r2 * s.c * inv_den;
+  g = s.c * inv_den + (r2 * s.c) * s.K / ((T)2 * sq * den * den);
+  const T r4 = r2 * r2;
+  T p = r4, q = (T)1;
+#pragma unroll
+  for (int i = 0; i < IACT_MAX_ASPHERIC; ++i) {
+    if (i < s.na) {
+      z += s.a[i] * p;
+      g += s.a[i] * (T)(2 * (i + 1)) * r2 * q;
+    }
+    p *= r4;
+    q *= r4;
+  }
+}
+
+// Local point (x, y, sag) and unit normal at (x, y) of a surface used with
+// offset (x0, y0) (surfaces.py:41-65).
+IACT_DEV void surface_point_normal(const SurfT<float>& s, float x, float y, float x0, float y0,
+                                   float z0, f3& p, f3& n) {
+  const float X = x + x0, Y = y + y0;
+  float z, g;
+  sag_and_dr2<float>(s, X * X + Y * Y, z, g);
+  p = mk3(x, y, z - z0);
+  const float zx = 2.f * X * g, zy = 2.f * Y * g;
+  const float inv = frsq(zx * zx + zy * zy + 1.f);
+  n = mk3(-zx * inv, -zy * inv, inv);
+}
+
+// Closed-form conic root (intersections.py:229-285), float64.
+IACT_DEV double intersect_conic(double ox, double oy, double oz, double dx, double dy, double dz,
+                                double c, double k) {
+  const double ok = 1.0 + k;
+  const double A = c * (dx * dx + dy * dy + ok * dz * dz);
+  const double B = 2.0 * (c * (ox * dx + oy * dy + ok * oz * dz) - dz);
+  const double C = c * (ox * ox + oy * oy + ok * oz * oz) - 2.0 * oz;
+  if (fabs(c) < 1e-12) return (fabs(dz) > 1e-10) ? -oz / dz : __builtin_inf();
+  const double disc = B * B - 4.0 * A * C;
+  if (disc < 0.0) return __builtin_inf();
+  const double sq = __builtin_sqrt(disc);
+  const double inv = 1.0 / (2.0 * A + 1e-30);
+  const double t1 = (-B - sq) * inv, t2 = (-B + sq) * inv;
+  const bool v1 = t1 > 1e-8, v2 = t2 > 1e-8;
+  return (v1 && v2) ? fmin(t1, t2) : (v1 ? t1 : (v2 ? t2 : __builtin_inf()));
+}
+
+// Fixed-count Newton (intersections.py:290-367) in float64 with the
+// reference's freeze rule: a lane stops updating one iteration after |g| < tol.
+// Iterations after every lane has frozen (or gone non-finite, which can never
+// become valid again) change nothing, so the loop leaves early when the whole
+// wave is done -- the result is identical to running all ten.
+IACT_DEV double newton_t(const SurfT<double>& s, double ox, double oy, double oz, double dx,
+                         double dy, double dz, double x0, double y0, double z0, double t,
+                         bool& valid) {
+  bool conv = false;
+  for (int it = 0; it < 10; ++it) {
+    const double X = ox + t * dx + x0, Y = oy + t * dy + y0;
+    double z, g;
+    sag_and_dr2<double>(s, X * X + Y * Y, z, g);
+    const double gv = (oz + t * dz) - (z - z0);
+    double gp = dz - (2.0 * X * g * dx + 2.0 * Y * g * dy);
+    if (!(fabs(gp) > 1e-12)) gp = 1e-12;
+    const double tn = t - gv / gp;
+    const bool newc = conv || (fabs(gv) < 1e-8);
+    if (!conv) t = tn;
+    conv = newc;
+    const bool done = conv || !__builtin_isfinite(t);
+    if (__all(done)) break;
+  }
+  const double X = ox + t * dx + x0, Y = oy + t * dy + y0;
+  const double res = fabs((oz + t * dz) - (sag_raw<double>(s, X, Y) - z0));
+  valid = (t > 1e-8) && (res < 1e-6);
+  return t;
+}
+
+// ---------------------------------------------------------------- obstructions
+// Each returns the ray parameter of the nearest hit or +inf
+// (intersections.py:44-226).  The shadow test only asks t < 1e10.
+IACT_DEV float hit_cylinder(f3 o, f3 d, const float* c) {
+  const float eps = 1e-8f;
+  const f3 p1 = mk3(c[0], c[1], c[2]);
+  const f3 ax = mk3(c[3], c[4], c[5]);
+  const float h = c[6], r = c[7];
+  const f3 oc = o - p1;
+  const float oca = dot(oc, ax), rda = dot(d, ax);
+  const f3 ocp = oc - oca * ax, rdp = d - rda * ax;
+  const float a = dot(rdp, rdp);
+  const float b = 2.f * dot(ocp, rdp);
+  const float cc = dot(ocp, ocp) - r * r;
+  const float disc = b * b - 4.f * a * cc;
+  const float sq = fsqrt(fmaxf(disc, 0.f));
+  const float ia = frcp(2.f * a + eps);
+  float t1 = (-b - sq) * ia, t2 = (-b + sq) * ia;
+  const float y1 = oca + t1 * rda, y2 = oca + t2 * rda;
+  const float inf = __builtin_inff();
+  t1 = (t1 > eps && y1 >= 0.f && y1 <= h && disc >= 0.f) ? t1 : inf;
+  t2 = (t2 > eps && y2 >= 0.f && y2 <= h && disc >= 0.f) ? t2 : inf;
+  const float ir = frcp(rda + eps);
+  float tb = -oca * ir, tt = (h - oca) * ir;
+  const f3 pb = ocp + tb * rdp, pt = ocp + tt * rdp;
+  tb = (tb > eps && dot(pb, pb) <= r * r) ? tb : inf;
+  tt = (tt > eps && dot(pt, pt) <= r * r) ? tt : inf;
+  return fminf(fminf(t1, t2), fminf(tb, tt));
+}
+
+IACT_DEV float hit_box(f3 o, f3 d, const float* b) {
+  const float eps = 1e-8f;
+  const float ix = frcp(d.x + eps), iy = frcp(d.y + eps), iz = frcp(d.z + eps);
+  const float tx1 = (b[0] - o.x) * ix, tx2 = (b[3] - o.x) * ix;
+  const float ty1 = (b[1] - o.y) * iy, ty2 = (b[4] - o.y) * iy;
+  const float tz1 = (b[2] - o.z) * iz, tz2 = (b[5] - o.z) * iz;
+  const float tmin = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+  const float tmax = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2));
+  const bool hit = (tmax >= tmin) && (tmax > eps);
+  return hit ? (tmin > eps ? tmin : tmax) : __builtin_inff();
+}
+
+IACT_DEV float hit_sphere(f3 o, f3 d, const float* s) {
+  const float eps = 1e-8f;
+  const f3 oc = o - mk3(s[0], s[1], s[2]);
+  const float a = dot(d, d), b = 2.f * dot(oc, d), c = dot(oc, oc) - s[3] * s[3];
+  const float disc = b * b - 4.f * a * c;
+  const float sq = fsqrt(fmaxf(disc, 0.f));
+  const float ia = frcp(2.f * a + eps);
+  const float t1 = (-b - sq) * ia, t2 = (-b + sq) * ia;
+  const float inf = __builtin_inff();
+  return fminf((t1 > eps && disc >= 0.f) ? t1 : inf, (t2 > eps && disc >= 0.f) ? t2 : inf);
+}
+
+IACT_DEV float signf1(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+
+IACT_DEV float hit_oriented_box(f3 o, f3 d, const float* q) {
+  const float eps = 1e-8f;
+  m3 R;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R.m[i] = q[6 + i];
+  const f3 lo = mulT(R, o - mk3(q[0], q[1], q[2]));
+  const f3 ld = mulT(R, d);
+  const float ix = frcp(ld.x + eps * signf1(ld.x + eps));
+  const float iy = frcp(ld.y + eps * signf1(ld.y + eps));
+  const float iz = frcp(ld.z + eps * signf1(ld.z + eps));
+  const float tx1 = (-q[3] - lo.x) * ix, tx2 = (q[3] - lo.x) * ix;
+  const float ty1 = (-q[4] - lo.y) * iy, ty2 = (q[4] - lo.y) * iy;
+  const float tz1 = (-q[5] - lo.z) * iz, tz2 = (q[5] - lo.z) * iz;
+  const float tmin = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+  const float tmax = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2));
+  const bool hit = (tmax >= tmin) && (tmax > eps);
+  const float tr = tmin > eps ? tmin : tmax;
+  return (hit && tr > eps) ? tr : __builtin_inff();
+}
+
+IACT_DEV float hit_triangle(f3 o, f3 d, const float* v) {
+  const float eps = 1e-8f;
+  const f3 v0 = mk3(v[0], v[1], v[2]);
+  const f3 e1 = mk3(v[3], v[4], v[5]) - v0;
+  const f3 e2 = mk3(v[6], v[7], v[8]) - v0;
+  const f3 h = cross(d, e2);
+  const float a = dot(e1, h);
+  const bool parallel = fabsf(a) < eps;
+  const float f = frcp(a + eps * signf1(a + eps));
+  const f3 s = o - v0;
+  const float u = f * dot(s, h);
+  const f3 q = cross(s, e1);
+  const float vv = f * dot(d, q);
+  const float t = f * dot(e2, q);
+  const bool ok = !parallel && u >= 0.f && u <= 1.f && vv >= 0.f && u + vv <= 1.f && t > eps;
+  return ok ? t : __builtin_inff();
+}
+
+// 0 if any obstruction has t < 1e10 along the unbounded ray (render.py:21-41).
+// Once every lane of the wave is blocked the remaining primitives cannot
+// change any result, so the test returns early for the whole wave.
+IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool active) {
+  const float big = 1e10f;
+  bool blocked = !active;
+  for (int i = 0; i < ob.n_cylinders; ++i) {
+    if (__all(blocked)) break;
+    blocked |= hit_cylinder(o, d, ob.cylinders + 8 * i) < big;
+  }
+  for (int i = 0; i < ob.n_boxes; ++i) {
+    if (__all(blocked)) break;
+    blocked |= hit_box(o, d, ob.boxes + 6 * i) < big;
+  }
+  for (int i = 0; i < ob.n_spheres; ++i) {
+    if (__all(blocked)) break;
+    blocked |= hit_sphere(o, d, ob.spheres + 4 * i) < big;
+  }
+  for (int i = 0; i < ob.n_oriented_boxes; ++i) {
+    if (__all(blocked)) break;
+    blocked |= hit_oriented_box(o, d, ob.oriented_boxes + 15 * i) < big;
+  }
+  for (int i = 0; i < ob.n_triangles; ++i) {
+    if (__all(blocked)) break;
+    blocked |= hit_triangle(o, d, ob.triangles + 9 * i) < big;
+  }
+  return (blocked && active) ? 0.f : 1.f;
+}
+
+IACT_DEV bool has_obstructions(const iact_obstructions& ob) {
+  return (ob.n_cylinders | ob.n_boxes | ob.n_spheres | ob.n_oriented_boxes | ob.n_triangles) != 0;
+}
+
+// ---------------------------------------------------------------- sensors
+// Derived, launch-constant sensor numbers.
+struct DevSensor {
+  int kind, splat, n_pixels;
+  f3 pos, u1, u2, nrm;
+  float ndotp;
+  // square
+  int width, height;
+  float x0, y0, dx, dy, inv_dx, inv_dy, wf, hf, edge_width;
+  // hex
+  float size, inv_size, size_s3, size_15, inv_inradius, gcos, gsin, gox, goy, edge_thr;
+  float qmin_f, rmin_f, tq_f, tr_f;
+  int q_min, r_min, tq, tr;
+  const int32_t* table;
+  // splat
+  float neg_half_inv_sigma2;
+  int ks;
+};
+
+// Sensor-plane intersection in the sensor frame (intersections.py:6-41);
+// returns false (and the 1e10 sentinel) for parallel rays or t <= 0.
+IACT_DEV void plane_hit(const DevSensor& S, f3 o, f3 d, float ndoto, float& x, float& y) {
+  const float ndotd = dot(d, S.nrm);
+  const bool parallel = fabsf(ndotd) < 1e-10f;
+  const float t = (S.ndotp - ndoto) * frcp(parallel ? 1.f : ndotd);
+  const f3 op = (o + t * d) - S.pos;
+  x = dot(op, S.u1);
+  y = dot(op, S.u2);
+  if (parallel || !(t > 0.f)) {
+    x = 1e10f;
+    y = 1e10f;
+  }
+}
+
+// Square binning (square.py:66-91): floor, range, edge distance.
+IACT_DEV int square_pixel(const DevSensor& S, float x, float y) {
+  const float xc = (x - S.x0) * S.inv_dx, yc = (y - S.y0) * S.inv_dy;
+  const float xf = floorf(xc), yf = floorf(yc);
+  bool ok = (xf >= 0.f) && (xf < S.wf) && (yf >= 0.f) && (yf < S.hf);
+  const float xr = xc - xf, yr = yc - yf;
+  const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
+  ok = ok && !(dist < S.edge_width);
+  return ok ? (int)yf * S.width + (int)xf : -1;
+}
+
+// World -> grid -> fractional axial (hexagonal.py:22-24, 146-150).
+IACT_DEV void hex_axial(const DevSensor& S, float x, float y, float& xg, float& yg, float& q,
+                        float& r) {
+  const float ux = x - S.gox, uy = y - S.goy;
+  xg = S.gcos * ux - S.gsin * uy;
+  yg = S.gsin * ux + S.gcos * uy;
+  const float third = 0.333333343267f;
+  q = (0.577350269189626f * xg - yg * third) * S.inv_size;
+  r = (2.f * yg * third) * S.inv_size;
+}
+
+// Cube rounding, half-to-even (hexagonal.py:32-39).
+IACT_DEV void hex_round(float q, float r, float& qi, float& ri) {
+  const float s = -q - r;
+  qi = rintf(q);
+  ri = rintf(r);
+  const float si = rintf(s);
+  const float dq = fabsf(qi - q), dr = fabsf(ri - r), ds = fabsf(si - s);
+  if (dq > dr && dq > ds)
+    qi = -ri - si;
+  else if (dr > dq && dr > ds)
+    ri = -qi - si;
+}
+
+// Table lookup with bounds (hexagonal.py:155-172); -1 when off-table/empty.
+IACT_DEV int hex_lookup(const DevSensor& S, const int32_t* table, float qi, float ri) {
+  const float qa = qi - S.qmin_f, ra = ri - S.rmin_f;
+  if (!(qa >= 0.f && qa < S.tq_f && ra >= 0.f && ra < S.tr_f)) return -1;
+  return table[(int)qa * S.tr + (int)ra];
+}
+
+IACT_DEV float hex_norm(const DevSensor& S, float x, float y) {
+  const float ax = fabsf(x);
+  return fmaxf(ax, 0.5f * ax + 0.866025403784439f * fabsf(y)) * S.inv_inradius;
+}
+
+// Hard hex binning (hexagonal.py:174-194).
+IACT_DEV int hex_pixel(const DevSensor& S, const int32_t* table, float x, float y) {
+  float xg, yg, q, r, qi, ri;
+  hex_axial(S, x, y, xg, yg, q, r);
+  hex_round(q, r, qi, ri);
+  const int pid = hex_lookup(S, table, qi, ri);
+  const float cx = S.size_s3 * (qi + ri * 0.5f), cy = S.size_15 * ri;
+  const float hd = hex_norm(S, xg - cx, yg - cy);
+  return (pid >= 0 && !(hd > S.edge_thr)) ? pid : -1;
+}
+
+// ---------------------------------------------------------------- wave ops
+IACT_DEV int lane_id() { return __lane_id(); }
+
+IACT_DEV float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+IACT_DEV double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Add every lane's value into img[pix] (pix < 0: nothing).  Lanes that share
+// a pixel are summed across the wave first so the LDS (or global) image sees
+// one atomic per distinct pixel; after `max_rounds` distinct pixels the rest
+// add directly.  Must be called by the whole wave in uniform control flow.
+template <bool GLOBAL>
+IACT_DEV void wave_accumulate(float* img, int pix, float v, int max_rounds) {
+  uint64_t rem = __ballot(pix >= 0 && v != 0.f);
+  int rounds = 0;
+  const int lane = lane_id();
+  while (rem) {
+    if (rounds == max_rounds) {
+      if ((rem >> lane) & 1ull) {
+        if (GLOBAL)
+          unsafeAtomicAdd(img + pix, v);
+        else
+          atomicAdd(img + pix, v);
+      }
+      break;
+    }
+    const int leader = __builtin_ctzll(rem);
+    const int key = __builtin_amdgcn_readlane(pix, leader);
+    const bool mine = ((rem >> lane) & 1ull) && pix == key;
+    const uint64_t m = __ballot(mine);
+    const float s = wave_sum(mine ? v : 0.f);
+    if (lane == leader) {
+      if (GLOBAL)
+        unsafeAtomicAdd(img + key, s);
+      else
+        atomicAdd(img + key, s);
+    }
+    rem &= ~m;
+    ++rounds;
+  }
+}
+
+}  // namespace iact

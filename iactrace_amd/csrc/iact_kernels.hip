@@ -1,0 +1,410 @@
+// Forward kernels: MC pupil sampling, world transform, image / debug trace
+// and response-matrix trace.  Host launchers at the bottom are called by the
+// C ABI in iact_abi.hip.
+#include "iact_kernels.h"
+
+namespace iact {
+
+// ------------------------------------------------------------------ utility
+__global__ void k_fill_f32(float* p, long long n, float v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_f64(double* p, long long n, double v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Ordered sum of per-block partial images: out[i] = sum_b partials[b][i].
+__global__ void k_reduce_partials(const float* partials, int nblocks, int P, float* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * P + i];
+    out[i] = s;
+  }
+}
+
+// ------------------------------------------------------------------ sampling
+// MCIntegrator.sample_group for one group (integrators.py:97-188) with the
+// build's Philox streams: stream 0 places the sample on the aperture
+// (sampling.py:8-67), stream 1 gives the two N(0,1) roughness draws
+// (reflection.py:22-50, Box-Muller).
+__global__ void k_sample_facets(iact_sample_desc D, float* points, float* normals, float* delta,
+                                float* weights) {
+  const long long total = (long long)D.n_facets * D.n_samples;
+  const SurfT<float> s = load_surface<float>(D.surface);
+  const uint32_t k0 = (uint32_t)D.seed, k1 = (uint32_t)(D.seed >> 32);
+  for (long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x; gi < total;
+       gi += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(gi / D.n_samples), m = (int)(gi % D.n_samples);
+    const u4 a = philox4x32_10(u4{(uint32_t)m, (uint32_t)f, (uint32_t)D.group, 0u}, k0, k1);
+    const u4 b = philox4x32_10(u4{(uint32_t)m, (uint32_t)f, (uint32_t)D.group, 1u}, k0, k1);
+    float x, y, area;
+    if (D.aperture == IACT_APERTURE_DISK) {
+      const float rad = D.radii[f];
+      const float r = __builtin_sqrtf(u01(a.x));
+      const float th = u01(a.y) * 2.f * 3.14159265358979323846f;
+      float sn, cs;
+      sincosf(th, &sn, &cs);
+      x = r * cs * rad;
+      y = r * sn * rad;
+      area = 3.14159265358979323846f * (rad * rad);
+    } else {
+      const float* v = D.vertices + (size_t)f * D.n_vertices * 2;
+      const int nt = D.n_vertices - 2;
+      // fan triangles from v0, area-weighted pick of the triangle
+      float cum[IACT_MAX_POLY_VERTS];
+      float acc = 0.f;
+      for (int i = 0; i < nt; ++i) {
+        const float ax = v[0], ay = v[1], bx = v[2 * (i + 1)], by = v[2 * (i + 1) + 1];
+        const float cx = v[2 * (i + 2)], cy = v[2 * (i + 2) + 1];
+        acc += fabsf((bx - ax) * (cy - ay) - (cx - ax) * (by - ay)) / 2.f;
+        cum[i] = acc;
+      }
+      const float target = u01(a.x) * cum[nt - 1];
+      int tri = 0;
+      for (int i = 0; i < nt - 1; ++i) tri += (target >= cum[i]) ? 1 : 0;
+      const float u = __builtin_sqrtf(u01(a.y)), vv = u01(a.z);
+      const float A = 1.f - u, B = u * (1.f - vv), C = u * vv;
+      x = A * v[0] + B * v[2 * (tri + 1)] + C * v[2 * (tri + 2)];
+      y = A * v[1] + B * v[2 * (tri + 1) + 1] + C * v[2 * (tri + 2) + 1];
+      float sh = 0.f;
+      for (int i = 0; i < D.n_vertices; ++i) {
+        const int j = (i + 1 == D.n_vertices) ? 0 : i + 1;
+        sh += v[2 * i] * v[2 * j + 1] - v[2 * j] * v[2 * i + 1];
+      }
+      area = 0.5f * fabsf(sh);
+    }
+    const float x0 = D.offsets[2 * f], y0 = D.offsets[2 * f + 1];
+    const float X = x + x0, Y = y + y0;
+    const float z0 = sag_raw<float>(s, x0, y0);
+    float z, g;
+    sag_and_dr2<float>(s, X * X + Y * Y, z, g);
+    const float zx = 2.f * X * g, zy = 2.f * Y * g;
+    const float inv = 1.f / __builtin_sqrtf(zx * zx + zy * zy + 1.f);
+    const f3 n = mk3(-zx * inv, -zy * inv, inv);
+    points[3 * gi] = x;
+    points[3 * gi + 1] = y;
+    points[3 * gi + 2] = z - z0;
+    normals[3 * gi] = n.x;
+    normals[3 * gi + 1] = n.y;
+    normals[3 * gi + 2] = n.z;
+    // roughness basis: ref axis x if |n.z| > 0.9 else z
+    const f3 ref = (fabsf(n.z) > 0.9f) ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 0.f, 1.f);
+    f3 t1 = cross(n, ref);
+    t1 = (1.f / __builtin_sqrtf(dot(t1, t1))) * t1;
+    const f3 t2 = cross(n, t1);
+    const float ua = u01_open(b.x), ub = u01(b.y);
+    const float rr = __builtin_sqrtf(-2.f * logf(ua));
+    float sb, cb;
+    sincosf(6.283185307179586f * ub, &sb, &cb);
+    const float th1 = rr * cb, th2 = rr * sb;
+    delta[3 * gi] = th1 * t1.x + th2 * t2.x;
+    delta[3 * gi + 1] = th1 * t1.y + th2 * t2.y;
+    delta[3 * gi + 2] = th1 * t1.z + th2 * t2.z;
+    weights[gi] = n.z / area * (float)D.n_samples;
+  }
+}
+
+// ------------------------------------------------------------------ world
+__global__ void k_facet_world(iact_facets F, DevSensor S, float* out) {
+  const long long total = (long long)F.n_facets * F.n_samples;
+  for (long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x; gi < total;
+       gi += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(gi / F.n_samples);
+    const m3 R = facet_rotation(F.rotations, f);
+    const f3 pos = mk3(F.positions[3 * f], F.positions[3 * f + 1], F.positions[3 * f + 2]);
+    f3 p, n;
+    world_sample(F.points, F.normals, F.delta, R, pos, F.perturbation_scale[f], (size_t)gi, p, n);
+    out[gi] = p.x;
+    out[total + gi] = p.y;
+    out[2 * total + gi] = p.z;
+    out[3 * total + gi] = n.x;
+    out[4 * total + gi] = n.y;
+    out[5 * total + gi] = n.z;
+    out[6 * total + gi] = F.weights[gi];
+  }
+}
+
+// ------------------------------------------------------------------ binning
+// Sensor.accumulate on given sensor-plane coordinates.
+template <int KIND, int SPLAT>
+__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_accumulate(DevSensor S, const float* x,
+                                                               const float* y, const float* v,
+                                                               long long n, float* out,
+                                                               int32_t* pixel) {
+  extern __shared__ float smem[];
+  int32_t* tab = (int32_t*)smem;
+  if (KIND == IACT_SENSOR_HEX) {
+    for (int i = threadIdx.x; i < S.tq * S.tr; i += blockDim.x) tab[i] = S.table[i];
+    __syncthreads();
+  }
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long nround = (n + stride - 1) / stride * stride;  // uniform trip count
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nround; i += stride) {
+    const bool active = i < n;
+    const float xi = active ? x[i] : 0.f, yi = active ? y[i] : 0.f, vi = active ? v[i] : 0.f;
+    if (!SPLAT) {
+      const int pix = active ? hard_pixel<KIND>(S, tab, xi, yi) : -1;
+      if (pixel && active) pixel[i] = pix;
+      wave_accumulate<true>(out, pix, vi, 16);
+    } else {
+      if (pixel && active) pixel[i] = hard_pixel<KIND>(S, tab, xi, yi);
+      splat_taps<KIND>(S, tab, xi, yi, active,
+                       [&](int pix, float w) { wave_accumulate<true>(out, pix, vi * w, 16); });
+    }
+  }
+}
+
+template <int KIND, int SPLAT>
+static hipError_t launch_acc_t(const DevSensor& S, const float* x, const float* y,
+                               const float* v, long long n, float* out, int32_t* pixel,
+                               hipStream_t st) {
+  const size_t lds = KIND == IACT_SENSOR_HEX ? (size_t)S.tq * S.tr * sizeof(int32_t) : 0;
+  long long g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL((k_accumulate<KIND, SPLAT>), dim3((int)g), dim3(256), lds, st, S, x, y, v, n,
+                     out, pixel);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const DevSensor& S, const float* x, const float* y, const float* v,
+                             long long n, float* out, int32_t* pixel, hipStream_t st) {
+  if (S.kind == IACT_SENSOR_SQUARE)
+    return S.splat ? launch_acc_t<IACT_SENSOR_SQUARE, 1>(S, x, y, v, n, out, pixel, st)
+                   : launch_acc_t<IACT_SENSOR_SQUARE, 0>(S, x, y, v, n, out, pixel, st);
+  return S.splat ? launch_acc_t<IACT_SENSOR_HEX, 1>(S, x, y, v, n, out, pixel, st)
+                 : launch_acc_t<IACT_SENSOR_HEX, 0>(S, x, y, v, n, out, pixel, st);
+}
+
+// ------------------------------------------------------------------ trace
+// Image / debug mode.  Work item = (source tile, chunk of 64 samples of one
+// facet); items are distributed over the waves of a persistent grid.
+template <int SRC, int KIND, int SPLAT, int MODE, bool STAGES>
+__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
+  extern __shared__ float smem[];
+  const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const int lane = lane_id();
+  const int P = A.S.n_pixels;
+  float* img_lds = smem;
+  int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? (A.det ? wpb : 1) * P : 0));
+  if (MODE == IACT_MODE_IMAGE && A.lds_image) {
+    const int n = (A.det ? wpb : 1) * P;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) img_lds[i] = 0.f;
+  }
+  if (KIND == IACT_SENSOR_HEX && A.lds_table) {
+    const int n = A.S.tq * A.S.tr;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
+  }
+  __syncthreads();
+  const int32_t* table = tab_lds;  // hex tables are always staged in LDS
+  float* img = img_lds + (A.det ? wave * P : 0);
+  const long long nchunks = (long long)A.nf * A.cpf;
+
+  for (long long item = (long long)blockIdx.x * wpb + wave; item < A.n_items;
+       item += (long long)gridDim.x * wpb) {
+    const long long chunk = item % nchunks;
+    const int stile = (int)(item / nchunks);
+    const int f = (int)(chunk / A.cpf);
+    const int m = (int)(chunk % A.cpf) * 64 + lane;
+    const bool active = m < A.M;
+    const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
+    const m3 R = facet_rotation(A.rotations, f);
+    const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
+    f3 pw, nw;
+    world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
+    const Sample smp = make_sample(A.S, pw, nw, A.weights[gi]);
+    const int s0 = stile * A.source_tile;
+    const int s1 = min(s0 + A.source_tile, A.ns);
+    for (int s = s0; s < s1; ++s) {
+      float x, y, v;
+      trace_ray<SRC, STAGES>(A, smp, active, A.sources[3 * s], A.sources[3 * s + 1],
+                             A.sources[3 * s + 2], A.values[s], x, y, v);
+      if (MODE == IACT_MODE_DEBUG) {
+        if (active) {
+          const size_t idx = ((size_t)f * A.ns + s) * A.M + m;
+          A.out[idx] = v;
+          A.dbg_xy[2 * idx] = x;
+          A.dbg_xy[2 * idx + 1] = y;
+          if (A.dbg_pix) A.dbg_pix[idx] = hard_pixel<KIND>(A.S, table, x, y);
+        }
+      } else if (!SPLAT) {
+        const int pix = active ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        if (A.lds_image)
+          wave_accumulate<false>(img, pix, v, A.max_rounds);
+        else
+          wave_accumulate<true>(A.out, pix, v, A.max_rounds);
+      } else {
+        splat_taps<KIND>(A.S, table, x, y, active, [&](int pix, float w) {
+          if (A.lds_image)
+            wave_accumulate<false>(img, pix, v * w, A.max_rounds);
+          else
+            wave_accumulate<true>(A.out, pix, v * w, A.max_rounds);
+        });
+      }
+    }
+  }
+
+  if (MODE == IACT_MODE_IMAGE && A.lds_image) {
+    __syncthreads();
+    if (A.det) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        float s = 0.f;
+        for (int w = 0; w < wpb; ++w) s += img_lds[w * P + i];
+        A.partials[(size_t)blockIdx.x * P + i] = s;
+      }
+    } else {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const float v = img_lds[i];
+        if (v != 0.f) unsafeAtomicAdd(A.out + i, v);
+      }
+    }
+  }
+}
+
+// Response matrix: each block owns one source at a time, accumulates its
+// image row in LDS over every (facet, sample), then writes the full row
+// (render.py:271-324).  World samples come from the precomputed SoA.
+template <int SRC, int KIND, int SPLAT>
+__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) {
+  extern __shared__ float smem[];
+  const int P = A.S.n_pixels;
+  float* row = smem;
+  int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? P : 0));
+  if (KIND == IACT_SENSOR_HEX && A.lds_table) {
+    const int n = A.S.tq * A.S.tr;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
+  }
+  const int32_t* table = tab_lds;
+  const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const int lane = lane_id();
+  const long long total = (long long)A.nf * A.M;
+  const long long nchunks = (long long)A.nf * A.cpf;
+  for (int s = blockIdx.x; s < A.ns; s += gridDim.x) {
+    if (A.lds_image)
+      for (int i = threadIdx.x; i < P; i += blockDim.x) row[i] = 0.f;
+    __syncthreads();
+    float* gdst = A.out + (size_t)s * P;
+    const double sx = A.sources[3 * s], sy = A.sources[3 * s + 1], sz = A.sources[3 * s + 2];
+    const float val = A.values[s];
+    for (long long c = wave; c < nchunks; c += wpb) {
+      const int f = (int)(c / A.cpf);
+      const int m = (int)(c % A.cpf) * 64 + lane;
+      const bool active = m < A.M;
+      const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
+      const f3 p = mk3(A.world[gi], A.world[total + gi], A.world[2 * total + gi]);
+      const f3 n = mk3(A.world[3 * total + gi], A.world[4 * total + gi], A.world[5 * total + gi]);
+      const Sample smp = make_sample(A.S, p, n, A.world[6 * total + gi]);
+      float x, y, v;
+      trace_ray<SRC>(A, smp, active, sx, sy, sz, val, x, y, v);
+      if (!SPLAT) {
+        const int pix = active ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        if (A.lds_image)
+          wave_accumulate<false>(row, pix, v, A.max_rounds);
+        else
+          wave_accumulate<true>(gdst, pix, v, A.max_rounds);
+      } else {
+        splat_taps<KIND>(A.S, table, x, y, active, [&](int pix, float w) {
+          if (A.lds_image)
+            wave_accumulate<false>(row, pix, v * w, A.max_rounds);
+          else
+            wave_accumulate<true>(gdst, pix, v * w, A.max_rounds);
+        });
+      }
+    }
+    __syncthreads();
+    if (A.lds_image)
+      for (int i = threadIdx.x; i < P; i += blockDim.x) A.out[(size_t)s * P + i] = row[i];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static int grid_for(long long n, int threads, int cap) {
+  long long g = (n + threads - 1) / threads;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+hipError_t launch_fill(float* p, long long n, float v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample(const iact_sample_desc& D, float* p, float* n, float* d, float* w,
+                         hipStream_t st) {
+  const long long total = (long long)D.n_facets * D.n_samples;
+  hipLaunchKernelGGL(k_sample_facets, dim3(grid_for(total, 256, 16384)), dim3(256), 0, st, D, p,
+                     n, d, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_world(const iact_facets& F, const DevSensor& S, float* out, hipStream_t st) {
+  const long long total = (long long)F.n_facets * F.n_samples;
+  hipLaunchKernelGGL(k_facet_world, dim3(grid_for(total, 256, 16384)), dim3(256), 0, st, F, S,
+                     out);
+  return hipGetLastError();
+}
+
+template <int SRC, int KIND, int SPLAT, int MODE>
+static hipError_t launch_trace_t(const TraceArgs& A, const LaunchShape& L, hipStream_t st) {
+  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0)
+    hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, false>), dim3(L.grid), dim3(L.threads),
+                       L.lds, st, A);
+  else
+    hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, true>), dim3(L.grid), dim3(L.threads),
+                       L.lds, st, A);
+  return hipGetLastError();
+}
+
+template <int SRC, int KIND, int SPLAT>
+static hipError_t launch_resp_t(const TraceArgs& A, const LaunchShape& L, hipStream_t st) {
+  hipLaunchKernelGGL((k_trace_response<SRC, KIND, SPLAT>), dim3(L.grid), dim3(L.threads), L.lds,
+                     st, A);
+  return hipGetLastError();
+}
+
+template <int SRC>
+static hipError_t dispatch_trace(const TraceArgs& A, const LaunchShape& L, int mode,
+                                 hipStream_t st) {
+  const bool sq = A.S.kind == IACT_SENSOR_SQUARE;
+  const bool sp = A.S.splat != 0;
+  if (mode == IACT_MODE_RESPONSE) {
+    if (sq) return sp ? launch_resp_t<SRC, IACT_SENSOR_SQUARE, 1>(A, L, st)
+                      : launch_resp_t<SRC, IACT_SENSOR_SQUARE, 0>(A, L, st);
+    return sp ? launch_resp_t<SRC, IACT_SENSOR_HEX, 1>(A, L, st)
+              : launch_resp_t<SRC, IACT_SENSOR_HEX, 0>(A, L, st);
+  }
+  if (mode == IACT_MODE_DEBUG) {
+    if (sq) return launch_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_DEBUG>(A, L, st);
+    return launch_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_DEBUG>(A, L, st);
+  }
+  if (sq) return sp ? launch_trace_t<SRC, IACT_SENSOR_SQUARE, 1, IACT_MODE_IMAGE>(A, L, st)
+                    : launch_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_IMAGE>(A, L, st);
+  return sp ? launch_trace_t<SRC, IACT_SENSOR_HEX, 1, IACT_MODE_IMAGE>(A, L, st)
+            : launch_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_IMAGE>(A, L, st);
+}
+
+hipError_t launch_trace(const TraceArgs& A, const LaunchShape& L, int source_type, int mode,
+                        hipStream_t st) {
+  return source_type == IACT_SOURCE_POINT ? dispatch_trace<IACT_SOURCE_POINT>(A, L, mode, st)
+                                          : dispatch_trace<IACT_SOURCE_PARALLEL>(A, L, mode, st);
+}
+
+hipError_t launch_reduce(const float* partials, int nblocks, int P, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(grid_for(P, 256, 1024)), dim3(256), 0, st, partials,
+                     nblocks, P, out);
+  return hipGetLastError();
+}
+
+}  // namespace iact

@@ -1,0 +1,253 @@
+// Per-ray forward trace shared by the forward, adjoint and batched kernels.
+//
+// Work decomposition (MI355X): one lane = one MC sample of one primary facet;
+// a wave holds 64 samples of the same facet, so the facet pose, the source
+// being traced, the obstruction tables and the stage>=1 mirror table are all
+// wave-uniform (scalar loads).  A wave loops over a tile of sources with the
+// sample's world-space point/normal kept in registers; the hit pixels of the
+// wave are merged before touching the LDS image.
+#pragma once
+
+#include "iact_device.h"
+
+namespace iact {
+
+struct TraceArgs {
+  // stage-0 facets (local frame samples + pose)
+  int nf, M, cpf;  // cpf = chunks of 64 samples per facet
+  const float* points;
+  const float* normals;
+  const float* delta;
+  const float* weights;
+  const float* positions;
+  const float* rotations;
+  const float* pscale;
+  // sources
+  const double* sources;
+  const float* values;
+  int ns;
+  int source_tile, n_stiles;
+  long long n_items;
+  // stage >= 1
+  int n_mirrors, n_stages;
+  int stages[8];
+  const float* mirrors;
+  int n_surfaces;
+  iact_surface surfaces[IACT_MAX_SURFACES];
+  iact_obstructions ob;
+  DevSensor S;
+  // outputs / scratch
+  float* out;
+  float* dbg_xy;
+  int32_t* dbg_pix;
+  float* partials;
+  const float* world;  // SoA (7, nf*M) when precomputed
+  int lds_image;       // image accumulated in LDS
+  int lds_table;       // hex table staged in LDS
+  int det;             // deterministic (wave-private images, ordered sums)
+  int max_rounds;
+};
+
+// One sample in world space, ready to be traced against many sources.
+struct Sample {
+  f3 p, n;
+  float rw;        // 1 / weight
+  double px, py, pz;
+  float ndoto;     // sensor normal . p (for the primary-only plane hit)
+};
+
+IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w) {
+  Sample s;
+  s.p = p;
+  s.n = n;
+  s.rw = frcp(w);
+  s.px = (double)p.x;
+  s.py = (double)p.y;
+  s.pz = (double)p.z;
+  s.ndoto = dot(p, S.nrm);
+  return s;
+}
+
+// MirrorGroup.transform_to_world (mirrors.py:64-79) for one sample.
+IACT_DEV void world_sample(const float* points, const float* normals, const float* delta,
+                           const m3& R, f3 pos, float scale, size_t gi, f3& pw, f3& nw) {
+  const f3 pl = mk3(points[3 * gi], points[3 * gi + 1], points[3 * gi + 2]);
+  const f3 nl = mk3(normals[3 * gi], normals[3 * gi + 1], normals[3 * gi + 2]);
+  const f3 dl = mk3(delta[3 * gi], delta[3 * gi + 1], delta[3 * gi + 2]);
+  pw = mul(R, pl) + pos;
+  const f3 nr = mul(R, nl);
+  const f3 dr = mul(R, dl);
+  const f3 pert = nr + scale * dr;
+  nw = frsq(dot(pert, pert)) * pert;
+}
+
+IACT_DEV m3 facet_rotation(const float* rotations, int f) {
+  return euler_to_matrix(rotations[3 * f], rotations[3 * f + 1], rotations[3 * f + 2]);
+}
+
+// Aperture test of a stage>=1 mirror (mirrors.py:147-149 disk,
+// 209-220 convex CCW polygon).
+IACT_DEV bool in_aperture(const float* M, float x, float y) {
+  if (M[17] == 0.f) return x * x + y * y <= M[18] * M[18];
+  const int n = (int)M[19];
+  bool ok = true;
+  for (int i = 0; i < n; ++i) {
+    const int j = (i + 1 == n) ? 0 : i + 1;
+    const float ax = M[20 + 2 * i], ay = M[21 + 2 * i];
+    const float bx = M[20 + 2 * j], by = M[21 + 2 * j];
+    ok = ok && ((bx - ax) * (y - ay) - (by - ay) * (x - ax) >= 0.f);
+  }
+  return ok;
+}
+
+// Hit of one stage>=1 mirror in its local frame (surfaces.py:67-108):
+// float64 conic guess + Newton, then fp32 point/normal at the rounded (x, y).
+struct MirrorHit {
+  float t;
+  f3 pl, nl;
+};
+
+IACT_DEV MirrorHit mirror_hit(const TraceArgs& A, const float* M, f3 ol, f3 dl) {
+  const iact_surface& su = A.surfaces[(int)M[38]];
+  const SurfT<double> s64 = load_surface<double>(su);
+  const double x0 = (double)M[15], y0 = (double)M[16];
+  const double z0 = sag_raw<double>(s64, x0, y0);
+  const double t0 = intersect_conic((double)ol.x + x0, (double)ol.y + y0, (double)ol.z + z0,
+                                    (double)dl.x, (double)dl.y, (double)dl.z, s64.c, s64.k);
+  bool ok;
+  const double t = newton_t(s64, ol.x, ol.y, ol.z, dl.x, dl.y, dl.z, x0, y0, z0, t0, ok);
+  MirrorHit h;
+  const float xh = (float)((double)ol.x + t * (double)dl.x);
+  const float yh = (float)((double)ol.y + t * (double)dl.y);
+  const SurfT<float> s32 = load_surface<float>(su);
+  const float z0f = sag_raw<float>(s32, M[15], M[16]);
+  surface_point_normal(s32, xh, yh, M[15], M[16], z0f, h.pl, h.nl);
+  h.t = (ok && in_aperture(M, xh, yh)) ? (float)t : __builtin_inff();
+  return h;
+}
+
+IACT_DEV m3 mirror_rotation(const float* M) {
+  m3 R;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R.m[i] = M[6 + i];
+  return R;
+}
+
+// _reflect_at_stage (render.py:44-79): nearest hit over the stage's mirrors
+// (first minimum wins, table is ordered by group then index), reflect, and
+// value *= hit * shadow * |cos|.  A miss keeps point 0 / normal 0.
+IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, float& v,
+                               bool active) {
+  float best = __builtin_inff();
+  f3 bp = mk3(0.f, 0.f, 0.f), bn = mk3(0.f, 0.f, 0.f);
+  for (int i = 0; i < A.n_mirrors; ++i) {
+    const float* M = A.mirrors + (size_t)IACT_MIRROR_STRIDE * i;
+    if ((int)M[36] != stage) continue;
+    const m3 R = mirror_rotation(M);
+    const f3 pos = mk3(M[0], M[1], M[2]);
+    const MirrorHit h = mirror_hit(A, M, mulT(R, o - pos), mulT(R, d));
+    if (h.t < best) {
+      best = h.t;
+      bp = mul(R, h.pl) + pos;
+      bn = mul(R, h.nl);
+    }
+  }
+  const float c = dot(d, bn);
+  const f3 r = d - (2.f * c) * bn;
+  const float hit = best < 1e10f ? 1.f : 0.f;
+  const float shadow = has_obstructions(A.ob) ? shadow_factor(A.ob, o, d, active) : 1.f;
+  v = v * hit * shadow * fabsf(c);
+  o = bp;
+  d = r;
+}
+
+// Trace one (source, sample) ray to the sensor plane (render.py:118-157).
+template <int SRC, bool STAGES = true>
+IACT_DEV void trace_ray(const TraceArgs& A, const Sample& s, bool active, double sx, double sy,
+                        double sz, float value, float& x, float& y, float& v) {
+  f3 d;
+  if (SRC == IACT_SOURCE_POINT) {
+    // p - src in float64 (sources sit at ~1e10 m), then normalised in fp32.
+    const float ddx = (float)(s.px - sx), ddy = (float)(s.py - sy), ddz = (float)(s.pz - sz);
+    const float inv = frsq(ddx * ddx + ddy * ddy + ddz * ddz);
+    d = mk3(ddx * inv, ddy * inv, ddz * inv);
+  } else {
+    d = mk3((float)sx, (float)sy, (float)sz);  // used as given (Q15)
+  }
+  const float shadow = has_obstructions(A.ob) ? shadow_factor(A.ob, s.p, -d, active) : 1.f;
+  const float c = dot(d, s.n);
+  f3 r = d - (2.f * c) * s.n;
+  v = value * (-c) * s.rw * shadow;
+  f3 o = s.p;
+  float ndoto = s.ndoto;
+  if (STAGES && A.n_stages > 0) {
+    for (int k = 0; k < A.n_stages; ++k) reflect_at_stage(A, A.stages[k], o, r, v, active);
+    ndoto = dot(o, A.S.nrm);
+  }
+  plane_hit(A.S, o, r, ndoto, x, y);
+  if (!active) v = 0.f;
+}
+
+template <int KIND>
+IACT_DEV int hard_pixel(const DevSensor& S, const int32_t* table, float x, float y) {
+  if (KIND == IACT_SENSOR_SQUARE) return square_pixel(S, x, y);
+  return hex_pixel(S, table, x, y);
+}
+
+// Splat taps of one ray (square.py:144-173, hexagonal.py:264-315).  Calls
+// fn(pixel, weight) for every tap; weights are normalised to sum 1.  Rays
+// with a non-finite coordinate or a zero normaliser contribute nothing.
+template <int KIND, typename Fn>
+IACT_DEV void splat_taps(const DevSensor& S, const int32_t* table, float x, float y, bool active,
+                         Fn&& fn) {
+  if (KIND == IACT_SENSOR_SQUARE) {
+    const float xp = (x - S.x0) * S.inv_dx, yp = (y - S.y0) * S.inv_dy;
+    const float xb = floorf(xp), yb = floorf(yp);
+    const float xf = xp - xb, yf = yp - yb;
+    const int ks = S.ks;
+    float tot = 0.f;
+    for (int oy = -ks; oy <= ks; ++oy)
+      for (int ox = -ks; ox <= ks; ++ox) {
+        const float ex = xf - (float)ox, ey = yf - (float)oy;
+        tot += __expf(S.neg_half_inv_sigma2 * (ex * ex + ey * ey));
+      }
+    const bool ok = active && tot > 0.f && fabsf(xb) < 1e8f && fabsf(yb) < 1e8f;
+    const float it = ok ? frcp(tot) : 0.f;
+    for (int oy = -ks; oy <= ks; ++oy)
+      for (int ox = -ks; ox <= ks; ++ox) {
+        const float ex = xf - (float)ox, ey = yf - (float)oy;
+        const float w = __expf(S.neg_half_inv_sigma2 * (ex * ex + ey * ey)) * it;
+        const float xi = xb + (float)ox, yi = yb + (float)oy;
+        const bool in = ok && xi >= 0.f && xi < S.wf && yi >= 0.f && yi < S.hf;
+        fn(in ? (int)yi * S.width + (int)xi : -1, w);
+      }
+  } else {
+    float xg, yg, q, r, qb, rb;
+    hex_axial(S, x, y, xg, yg, q, r);
+    hex_round(q, r, qb, rb);
+    const float bx = S.size_s3 * (qb + rb * 0.5f), by = S.size_15 * rb;
+    const float ddx = xg - bx, ddy = yg - by;
+    const int ks = S.ks;
+    float tot = 0.f;
+    for (int oq = -ks; oq <= ks; ++oq)
+      for (int orr = -ks; orr <= ks; ++orr) {
+        if (max(max(abs(oq), abs(orr)), abs(oq + orr)) > ks) continue;
+        const float ox = S.size_s3 * ((float)oq + (float)orr * 0.5f), oy = S.size_15 * (float)orr;
+        const float hd = hex_norm(S, ddx - ox, ddy - oy);
+        tot += __expf(S.neg_half_inv_sigma2 * hd * hd);
+      }
+    const bool ok = active && tot > 0.f && fabsf(qb) < 1e8f && fabsf(rb) < 1e8f;
+    const float it = ok ? frcp(tot) : 0.f;
+    for (int oq = -ks; oq <= ks; ++oq)
+      for (int orr = -ks; orr <= ks; ++orr) {
+        if (max(max(abs(oq), abs(orr)), abs(oq + orr)) > ks) continue;
+        const float ox = S.size_s3 * ((float)oq + (float)orr * 0.5f), oy = S.size_15 * (float)orr;
+        const float hd = hex_norm(S, ddx - ox, ddy - oy);
+        const float w = __expf(S.neg_half_inv_sigma2 * hd * hd) * it;
+        const int pid = ok ? hex_lookup(S, table, qb + (float)oq, rb + (float)orr) : -1;
+        fn(pid, w);
+      }
+  }
+}
+
+}  // namespace iact

@@ -1,0 +1,111 @@
+"""Test helpers: convert product telescopes into oracle scenes and compare.
+
+The oracle (oracle/iact_oracle.py) is an independent numpy restatement of
+the reference; these helpers only move data between the two.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+import oracle.iact_oracle as O
+from iactrace_amd.mirrors import AsphericDiskMirrorGroup
+from iactrace_amd.sensors import (DifferentiableHexagonalSensor, DifferentiableSquareSensor,
+                                  HexagonalSensor, SquareSensor)
+
+
+def np32(t):
+    return t.detach().float().cpu().numpy()
+
+
+def oracle_sensor(s):
+    if isinstance(s, SquareSensor):
+        spec = O.SquareSensorSpec(tuple(map(float, s.position)), tuple(map(float, s.rotation)),
+                                  s.width, s.height, s.bounds, s.edge_width)
+        if isinstance(s, DifferentiableSquareSensor):
+            spec.splat, spec.sigma, spec.kernel_size = True, s.sigma, s.kernel_size
+        return spec
+    assert isinstance(s, HexagonalSensor)
+    spec = O.HexSensorSpec(tuple(map(float, s.position)), tuple(map(float, s.rotation)),
+                           np.asarray(s.hex_centers, np.float32), s.edge_width)
+    if isinstance(s, DifferentiableHexagonalSensor):
+        spec.splat, spec.sigma, spec.kernel_size = True, s.sigma, s.kernel_size
+    return spec
+
+
+def oracle_obstructions(groups):
+    ob = O.Obstructions()
+    for g in groups or []:
+        k = g.kind
+        if k == "cylinder":
+            ob.cylinders += [(g.p1[i], g.p2[i], float(g.r[i])) for i in range(len(g))]
+        elif k == "box":
+            ob.boxes += [(g.p1[i], g.p2[i]) for i in range(len(g))]
+        elif k == "sphere":
+            ob.spheres += [(g.centers[i], float(g.radii[i])) for i in range(len(g))]
+        elif k == "oriented_box":
+            ob.oriented_boxes += [(g.centers[i], g.half_extents[i], g.rotations[i])
+                                  for i in range(len(g))]
+        elif k == "triangle":
+            ob.triangles += [(g.v0[i], g.v1[i], g.v2[i]) for i in range(len(g))]
+    return ob
+
+
+def oracle_scene(tel, sensor_idx=0):
+    prim = [g for g in tel.mirror_groups if g.optical_stage == 0]
+    sec = [g for g in tel.mirror_groups if g.optical_stage > 0]
+    mirrors = []
+    stage_groups = {}
+    for g in sec:
+        gi = stage_groups.setdefault(g.optical_stage, 0)
+        stage_groups[g.optical_stage] += 1
+        surf = O.Surface(g.curvature, g.conic, g.aspheric.detach().double().numpy())
+        for i in range(len(g)):
+            kw = dict(position=tuple(np32(g.positions[i])), rotation=tuple(np32(g.rotations[i])),
+                      surface=surf, offset=tuple(np32(g.offsets[i])), stage=g.optical_stage,
+                      group=gi)
+            if isinstance(g, AsphericDiskMirrorGroup):
+                kw["radius"] = float(g.radii[i])
+            else:
+                kw["vertices"] = np32(g.vertices[i])
+            mirrors.append(O.MirrorSpec(**kw))
+    cat = lambda name: np.concatenate([np32(getattr(g, name)) for g in prim], 0)  # noqa: E731
+    return O.Scene(points=cat("points"), normals=cat("normals"),
+                   delta=cat("perturbation_delta"), weights=cat("weights")[..., 0],
+                   positions=cat("positions"), rotations=cat("rotations"),
+                   scale=np.concatenate([np32(g.perturbation_scale) for g in prim]),
+                   sensor=oracle_sensor(tel.sensors[sensor_idx]),
+                   obstructions=oracle_obstructions(tel.obstruction_groups), mirrors=mirrors)
+
+
+def grown_obstructions(ob, eps):
+    """Every obstruction grown by eps (eps < 0 shrinks) -- defines the shadow band."""
+    g = O.Obstructions()
+    g.cylinders = [(p1, p2, r + eps) for p1, p2, r in ob.cylinders]
+    g.boxes = [(np.minimum(a, b) - eps, np.maximum(a, b) + eps) for a, b in ob.boxes]
+    g.spheres = [(c, r + eps) for c, r in ob.spheres]
+    g.oriented_boxes = [(c, np.asarray(h) + eps, R) for c, h, R in ob.oriented_boxes]
+    g.triangles = list(ob.triangles)
+    return g
+
+
+def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
+    """Rays (facet-major debug order) whose fate an fp32 rounding can flip:
+    sensor coordinate within px_band pixels (or hex-norm units) of a binning
+    boundary, or a shadow/aperture decision that changes when obstructions and
+    stage>=1 apertures are grown or shrunk by ob_eps metres (float64 oracle).
+    """
+    x, y, v, _ = O.render(sc, sources, values, source_type, "fp64", "debug")
+    band = O.edge_margin(x, y, sc.sensor) < px_band
+    alive = v != 0
+    import copy
+    for eps in (ob_eps, -ob_eps):
+        sc2 = copy.copy(sc)
+        sc2.obstructions = grown_obstructions(sc.obstructions, eps)
+        sc2.mirrors = [copy.copy(m) for m in sc.mirrors]
+        for m in sc2.mirrors:
+            if m.radius is not None:
+                m.radius = m.radius - eps
+        _, _, v2, _ = O.render(sc2, sources, values, source_type, "fp64", "debug")
+        band |= (v2 != 0) != alive
+    return band, x, y, v
