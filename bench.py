@@ -1,0 +1,189 @@
+"""Benchmark of the MI355X ray-tracing path (one JSON line on stdout).
+
+Headline: configuration C2 -- Davies-Cotton dish (382 hexagonal facets,
+f = 15 m) x 4096 MC samples per facet x 4096 point sources at 1e10 m, square
+32x30 camera -- plain forward render, rays/s.  A step is one render of the
+whole workload; sources, values and samples are resident in HBM before the
+timed region starts.
+
+Also reported: the roofline of the trace kernel (algorithmic flops per launch
+over its HIP-event duration, against MI355X's fp32 vector peak), HBM traffic
+from the committed rocprofv3 counter summary if present, and a CPU baseline:
+the oracle's C/OpenMP port timed on the host cores on a bounded source subsample.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+METRIC = "rays/s fwd and fwd+bwd, batched vs plain, 1 MI355X at ≤1e-5 image rel err"
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def time_steps(fn, steps, warmup):
+    """Wall time of `steps` calls and the HIP-event time of each call."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    _sync()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    _sync()
+    wall = time.perf_counter() - t0
+    ev_ms = [a.elapsed_time(b) for a, b in evs]
+    return wall, ev_ms
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(wl, tel, target_s=15.0):
+    """Time the C/OpenMP port on a source subsample of the same workload."""
+    import numpy as np
+    from oracle import cpu_ref
+    from tests.helpers import oracle_scene
+    sc = oracle_scene(tel)
+    keep = []
+    s = cpu_ref.scene_struct(sc, keep)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    P = sc.sensor.n_pixels
+    rays_per_src = sc.points.shape[0] * sc.points.shape[1]
+    probe = min(16, len(wl.sources))
+    t0 = time.perf_counter()
+    cpu_ref.render(s, wl.sources[:probe], wl.values[:probe], wl.source_type, P, nthreads)
+    dt = time.perf_counter() - t0
+    rate = probe * rays_per_src / max(dt, 1e-9)
+    n = int(max(probe, min(len(wl.sources), target_s * rate / rays_per_src)))
+    t0 = time.perf_counter()
+    img = cpu_ref.render(s, wl.sources[:n], wl.values[:n], wl.source_type, P, nthreads)
+    dt = time.perf_counter() - t0
+    return {"value": n * rays_per_src / dt, "unit": "rays/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{wl.name}: first {n} of {len(wl.sources)} sources x all "
+                      f"{sc.points.shape[0]} facets x {sc.points.shape[1]} samples "
+                      f"({n * rays_per_src:.3g} rays, {dt:.2f} s), oracle/cpu_ref.c fp32 "
+                      f"with fp64 directions, gcc -O3 -march=x86-64-v3 -fopenmp"}, img, n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (for counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path would shard sources across GPUs with one image all-reduce)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import iactrace_amd as ia
+    from iactrace_amd import configs
+    from iactrace_amd.flop_model import split_per_ray
+
+    dev = torch.device("cuda:0")
+    wl = configs.workload(args.config)
+    tel = wl.telescope(dev, key=0)
+    src = torch.as_tensor(wl.sources, dtype=torch.float64, device=dev)
+    vals = torch.as_tensor(wl.values, dtype=torch.float32, device=dev)
+    _sync()
+
+    def step():
+        return ia.render(tel, src, vals, wl.source_type)
+
+    wall, ev_ms = time_steps(step, args.steps, args.warmup)
+    n_rays = wl.n_rays
+    ms_per_step = 1e3 * wall / args.steps
+    value = n_rays * args.steps / wall
+    kern_ms = float(np.mean(ev_ms))
+    ob = wl.obstructions
+    n_cyl = sum(isinstance(o, ia.Cylinder) for o in ob)
+    n_box = sum(isinstance(o, ia.Box) for o in ob)
+    n_sph = sum(isinstance(o, ia.Sphere) for o in ob)
+    f32, f64 = split_per_ray(wl.source_type, n_cyl, n_box, n_sph,
+                             sum(m.optical_stage > 0 for m in wl.mirrors), 0,
+                             "square" if isinstance(wl.sensor, ia.SquareSensor) else "hex",
+                             len(wl.sources))
+    flops = n_rays * (f32 + f64)
+    achieved = flops / (kern_ms * 1e-3)
+    traffic = traffic_from_profiles(args.config)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 source directions)",
+        "data": "synthetic: generated dish geometry, Philox MC samples (key 0), "
+                "numpy PCG64 source field (seed 2)",
+        "config": {"workload": f"{wl.name} plain forward: Davies-Cotton f=15 m, "
+                               f"{wl.n_facets} hex facets x {wl.n_samples} samples, "
+                               f"{len(wl.sources)} point sources at 1e10 m (2.5 deg field, "
+                               f"lognormal values), square 32x30 camera",
+                   "rays_per_step": n_rays, "facets": wl.n_facets, "samples": wl.n_samples,
+                   "sources": len(wl.sources), "mode": "image, plain (not batched), forward"},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": traffic,
+                     "kernel": "k_trace (fill + trace per step, HIP events)",
+                     "kernel_ms": kern_ms,
+                     "flops_per_ray": {"fp32": f32, "fp64": f64},
+                     "fp64_frac_of_fp64_peak": n_rays * f64 / (kern_ms * 1e-3) / PEAK_FP64},
+    }
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        _sync()
+        b.copy_(a)
+        _sync()
+        out["copy_calibration_bytes"] = {"fetch": a.numel() * 4, "write": b.numel() * 4}
+    if not args.no_cpu:
+        cb, cimg, n = cpu_baseline(wl, tel)
+        out["cpu_baseline"] = cb
+        # sanity: the same subsample on the GPU agrees with the CPU port
+        g = ia.render(tel, src[:n], vals[:n], wl.source_type).detach().cpu().numpy().reshape(-1)
+        out["cpu_baseline"]["gpu_vs_cpu_image_l1_rel"] = float(
+            np.abs(g - cimg).sum() / max(np.abs(cimg).sum(), 1e-30))
+        out["speedup_vs_cpu_baseline"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,7 +81,8 @@ class IactScene(Structure):
 
 class IactLaunch(Structure):
     _fields_ = [("block_threads", c_int32), ("source_tile", c_int32),
-                ("blocks_per_cu", c_int32), ("deterministic", c_int32)]
+                ("blocks_per_cu", c_int32), ("deterministic", c_int32),
+                ("accumulate", c_int32)]
 
 
 class IactGrads(Structure):

@@ -156,14 +156,20 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
   S.u2 = f3{R[1], R[4], R[7]};
   S.nrm = f3{R[2], R[5], R[8]};
   S.ndotp = S.nrm.x * S.pos.x + S.nrm.y * S.pos.y + S.nrm.z * S.pos.z;
+  S.axis = (R[0] == 1.f && R[1] == 0.f && R[2] == 0.f && R[3] == 0.f && R[4] == 1.f &&
+            R[5] == 0.f && R[6] == 0.f && R[7] == 0.f && R[8] == 1.f)
+               ? 1
+               : 0;
   S.width = s.width;
   S.height = s.height;
   S.x0 = s.x0;
   S.y0 = s.y0;
   S.dx = s.dx;
   S.dy = s.dy;
-  S.inv_dx = s.dx > 0.f ? 1.f / s.dx : 0.f;
-  S.inv_dy = s.dy > 0.f ? 1.f / s.dy : 0.f;
+  S.inv_dx = s.dx > 0.f ? (float)(1.0 / (double)s.dx) : 0.f;
+  S.inv_dy = s.dy > 0.f ? (float)(1.0 / (double)s.dy) : 0.f;
+  S.nx0 = s.dx > 0.f ? (float)(-(double)s.x0 / (double)s.dx) : 0.f;
+  S.ny0 = s.dy > 0.f ? (float)(-(double)s.y0 / (double)s.dy) : 0.f;
   S.wf = (float)s.width;
   S.hf = (float)s.height;
   S.edge_width = s.edge_width;
@@ -176,6 +182,21 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
   S.gsin = sinf(-s.grid_rotation);
   S.gox = s.grid_offset[0];
   S.goy = s.grid_offset[1];
+  if (s.hex_size > 0.f) {
+    // q = (x_g / sqrt3 - y_g / 3) / size, r = (2 y_g / 3) / size with
+    // (x_g, y_g) = rotate(x - ox, y - oy, -grid_rotation)
+    const double gc = cos(-(double)s.grid_rotation), gs = sin(-(double)s.grid_rotation);
+    const double sz = s.hex_size, s33 = 0.5773502691896257;
+    const double aqx = (s33 * gc - gs / 3.0) / sz, aqy = (-s33 * gs - gc / 3.0) / sz;
+    const double arx = (2.0 / 3.0) * gs / sz, ary = (2.0 / 3.0) * gc / sz;
+    const double ox = s.grid_offset[0], oy = s.grid_offset[1];
+    S.aqx = (float)aqx;
+    S.aqy = (float)aqy;
+    S.aq0 = (float)(-(aqx * ox + aqy * oy));
+    S.arx = (float)arx;
+    S.ary = (float)ary;
+    S.ar0 = (float)(-(arx * ox + ary * oy));
+  }
   S.edge_thr = s.edge_threshold;
   S.q_min = s.q_min;
   S.r_min = s.r_min;
@@ -227,14 +248,16 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
     return set_error(IACT_ERR_INVALID, "block_threads %d must be a multiple of 64 in [64,%d]",
                      threads, IACT_MAX_BLOCK);
   const int wpb = threads / 64;
-  const int bpc = (L && L->blocks_per_cu) ? L->blocks_per_cu : 4;
+  const int bpc = (L && L->blocks_per_cu) ? L->blocks_per_cu : 8;
   if (bpc < 1 || bpc > 64) return set_error(IACT_ERR_INVALID, "blocks_per_cu %d", bpc);
-  int ts = (L && L->source_tile) ? L->source_tile : 64;
+  int ts = (L && L->source_tile) ? L->source_tile : 256;
   if (ts < 1) return set_error(IACT_ERR_INVALID, "source_tile %d", ts);
   if (ts > ns) ts = ns > 0 ? ns : 1;
   A.source_tile = ts;
   A.n_stiles = ns > 0 ? (ns + ts - 1) / ts : 0;
   A.det = (L && L->deterministic) ? 1 : 0;
+  A.acc = (L && !A.det) ? L->accumulate : 0;
+  if (A.acc < 0 || A.acc > 2) return set_error(IACT_ERR_INVALID, "accumulate mode %d", A.acc);
   const int P = sc.sensor.n_pixels;
   const size_t tab = sc.sensor.kind == IACT_SENSOR_HEX
                          ? (size_t)sc.sensor.table_q * sc.sensor.table_r * sizeof(int32_t)

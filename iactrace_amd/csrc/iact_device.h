@@ -14,6 +14,15 @@
 
 namespace iact {
 
+// Read-only view in the constant address space: wave-uniform loads through
+// it become scalar (s_load) loads served by the scalar cache.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+IACT_DEV cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+
 // ---------------------------------------------------------------- vectors
 struct f3 {
   float x, y, z;
@@ -23,7 +32,10 @@ IACT_DEV f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; 
 IACT_DEV f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 IACT_DEV f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
 IACT_DEV f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
-IACT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+IACT_DEV float dot(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+IACT_DEV f3 fma3(float s, f3 a, f3 b) {  // s * a + b
+  return f3{fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z)};
+}
 IACT_DEV f3 cross(f3 a, f3 b) {
   return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
@@ -33,12 +45,14 @@ struct m3 {
   float m[9];
 };
 IACT_DEV f3 mul(const m3& R, f3 v) {  // R v
-  return f3{R.m[0] * v.x + R.m[1] * v.y + R.m[2] * v.z, R.m[3] * v.x + R.m[4] * v.y + R.m[5] * v.z,
-            R.m[6] * v.x + R.m[7] * v.y + R.m[8] * v.z};
+  return f3{fmaf(R.m[0], v.x, fmaf(R.m[1], v.y, R.m[2] * v.z)),
+            fmaf(R.m[3], v.x, fmaf(R.m[4], v.y, R.m[5] * v.z)),
+            fmaf(R.m[6], v.x, fmaf(R.m[7], v.y, R.m[8] * v.z))};
 }
 IACT_DEV f3 mulT(const m3& R, f3 v) {  // R^T v
-  return f3{R.m[0] * v.x + R.m[3] * v.y + R.m[6] * v.z, R.m[1] * v.x + R.m[4] * v.y + R.m[7] * v.z,
-            R.m[2] * v.x + R.m[5] * v.y + R.m[8] * v.z};
+  return f3{fmaf(R.m[0], v.x, fmaf(R.m[3], v.y, R.m[6] * v.z)),
+            fmaf(R.m[1], v.x, fmaf(R.m[4], v.y, R.m[7] * v.z)),
+            fmaf(R.m[2], v.x, fmaf(R.m[5], v.y, R.m[8] * v.z))};
 }
 
 // Fast hardware approximations (about 1 ulp).  The reference divides and
@@ -344,8 +358,11 @@ IACT_DEV bool has_obstructions(const iact_obstructions& ob) {
 // Derived, launch-constant sensor numbers.
 struct DevSensor {
   int kind, splat, n_pixels;
+  int axis;        // frame is the identity (u1 = x, u2 = y, normal = z)
   f3 pos, u1, u2, nrm;
   float ndotp;
+  float nx0, ny0;  // -x0/dx, -y0/dy: pixel coordinate = fma(x, 1/dx, -x0/dx)
+  float aqx, aqy, aq0, arx, ary, ar0;  // sensor (x, y) -> fractional axial (q, r)
   // square
   int width, height;
   float x0, y0, dx, dy, inv_dx, inv_dy, wf, hf, edge_width;
@@ -359,41 +376,57 @@ struct DevSensor {
   int ks;
 };
 
-// Sensor-plane intersection in the sensor frame (intersections.py:6-41);
-// returns false (and the 1e10 sentinel) for parallel rays or t <= 0.
-IACT_DEV void plane_hit(const DevSensor& S, f3 o, f3 d, float ndoto, float& x, float& y) {
-  const float ndotd = dot(d, S.nrm);
-  const bool parallel = fabsf(ndotd) < 1e-10f;
-  const float t = (S.ndotp - ndoto) * frcp(parallel ? 1.f : ndotd);
-  const f3 op = (o + t * d) - S.pos;
-  x = dot(op, S.u1);
-  y = dot(op, S.u2);
-  if (parallel || !(t > 0.f)) {
-    x = 1e10f;
-    y = 1e10f;
+// Sensor-plane intersection (intersections.py:6-41) split into per-origin
+// constants -- kt = n.(c - o), kx = (o - c).u1, ky = (o - c).u2 -- and the per-
+// ray part x = kx + t (r.u1), y = ky + t (r.u2), t = kt / (r.n).  For the
+// primary leg the origin is the facet sample, so the constants are computed
+// once per sample and reused for every source.
+struct PlaneK {
+  float kt, kx, ky;
+};
+IACT_DEV PlaneK plane_consts(const DevSensor& S, f3 o) {
+  const f3 q = o - S.pos;
+  return PlaneK{-dot(q, S.nrm), dot(q, S.u1), dot(q, S.u2)};
+}
+// Returns false for rays parallel to the plane or hitting it at t <= 0, the
+// reference's 1e10 sentinel cases.
+template <bool AXIS>
+IACT_DEV bool plane_xy(const DevSensor& S, const PlaneK& K, f3 r, float& x, float& y) {
+  float nd, ru, rv;
+  if (AXIS) {
+    nd = r.z;
+    ru = r.x;
+    rv = r.y;
+  } else {
+    nd = dot(r, S.nrm);
+    ru = dot(r, S.u1);
+    rv = dot(r, S.u2);
   }
+  const float t = K.kt * frcp(nd);
+  x = fmaf(t, ru, K.kx);
+  y = fmaf(t, rv, K.ky);
+  return !(fabsf(nd) < 1e-10f) && t > 0.f;
 }
 
 // Square binning (square.py:66-91): floor, range, edge distance.
 IACT_DEV int square_pixel(const DevSensor& S, float x, float y) {
-  const float xc = (x - S.x0) * S.inv_dx, yc = (y - S.y0) * S.inv_dy;
+  const float xc = fmaf(x, S.inv_dx, S.nx0), yc = fmaf(y, S.inv_dy, S.ny0);
   const float xf = floorf(xc), yf = floorf(yc);
   bool ok = (xf >= 0.f) && (xf < S.wf) && (yf >= 0.f) && (yf < S.hf);
-  const float xr = xc - xf, yr = yc - yf;
-  const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
-  ok = ok && !(dist < S.edge_width);
+  if (S.edge_width > 0.f) {
+    const float xr = xc - xf, yr = yc - yf;
+    const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
+    ok = ok && !(dist < S.edge_width);
+  }
   return ok ? (int)yf * S.width + (int)xf : -1;
 }
 
-// World -> grid -> fractional axial (hexagonal.py:22-24, 146-150).
-IACT_DEV void hex_axial(const DevSensor& S, float x, float y, float& xg, float& yg, float& q,
-                        float& r) {
-  const float ux = x - S.gox, uy = y - S.goy;
-  xg = S.gcos * ux - S.gsin * uy;
-  yg = S.gsin * ux + S.gcos * uy;
-  const float third = 0.333333343267f;
-  q = (0.577350269189626f * xg - yg * third) * S.inv_size;
-  r = (2.f * yg * third) * S.inv_size;
+// Sensor (x, y) -> fractional axial (q, r): the grid offset, the grid
+// rotation and the pointy-top axial map (hexagonal.py:22-24, 146-150) folded
+// into one affine map.
+IACT_DEV void hex_axial(const DevSensor& S, float x, float y, float& q, float& r) {
+  q = fmaf(x, S.aqx, fmaf(y, S.aqy, S.aq0));
+  r = fmaf(x, S.arx, fmaf(y, S.ary, S.ar0));
 }
 
 // Cube rounding, half-to-even (hexagonal.py:32-39).
@@ -416,29 +449,43 @@ IACT_DEV int hex_lookup(const DevSensor& S, const int32_t* table, float qi, floa
   return table[(int)qa * S.tr + (int)ra];
 }
 
-IACT_DEV float hex_norm(const DevSensor& S, float x, float y) {
-  const float ax = fabsf(x);
-  return fmaxf(ax, 0.5f * ax + 0.866025403784439f * fabsf(y)) * S.inv_inradius;
+// Hexagonal norm (hexagonal.py:42-47) of the offset from the cell centre,
+// in units of the inradius, written in axial offsets (dq, dr):
+// x/inradius = 2 (dq + dr/2), y/inradius = sqrt(3) dr.
+IACT_DEV float hex_norm_axial(float dq, float dr) {
+  const float a = fabsf(fmaf(0.5f, dr, dq)), b = fabsf(dr);
+  return fmaxf(2.f * a, fmaf(1.5f, b, a));
 }
 
 // Hard hex binning (hexagonal.py:174-194).
 IACT_DEV int hex_pixel(const DevSensor& S, const int32_t* table, float x, float y) {
-  float xg, yg, q, r, qi, ri;
-  hex_axial(S, x, y, xg, yg, q, r);
+  float q, r, qi, ri;
+  hex_axial(S, x, y, q, r);
   hex_round(q, r, qi, ri);
   const int pid = hex_lookup(S, table, qi, ri);
-  const float cx = S.size_s3 * (qi + ri * 0.5f), cy = S.size_15 * ri;
-  const float hd = hex_norm(S, xg - cx, yg - cy);
+  const float hd = hex_norm_axial(q - qi, r - ri);
   return (pid >= 0 && !(hd > S.edge_thr)) ? pid : -1;
 }
 
 // ---------------------------------------------------------------- wave ops
 IACT_DEV int lane_id() { return __lane_id(); }
 
+// Cross-lane sum of a wave64 with DPP row operations (no LDS round trips):
+// pairs, quads, half rows, rows, then row broadcasts; lane 63 ends up with
+// the total, which is returned to every lane as a scalar.
+template <int CTRL, int ROW_MASK = 0xf>
+IACT_DEV float dpp_f(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xf, true));
+}
 IACT_DEV float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f<0xB1>(v);         // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);         // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);        // row_half_mirror
+  v += dpp_f<0x140>(v);        // row_mirror
+  v += dpp_f<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+  v += dpp_f<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 IACT_DEV double wave_sum_d(double v) {
 #pragma unroll

@@ -183,10 +183,10 @@ hipError_t launch_accumulate(const DevSensor& S, const float* x, const float* y,
 // ------------------------------------------------------------------ trace
 // Image / debug mode.  Work item = (source tile, chunk of 64 samples of one
 // facet); items are distributed over the waves of a persistent grid.
-template <int SRC, int KIND, int SPLAT, int MODE, bool STAGES>
+template <int SRC, int KIND, int SPLAT, int MODE, bool STAGES, bool AXIS>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   extern __shared__ float smem[];
-  const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
   const int lane = lane_id();
   const int P = A.S.n_pixels;
   float* img_lds = smem;
@@ -204,6 +204,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   float* img = img_lds + (A.det ? wave * P : 0);
   const long long nchunks = (long long)A.nf * A.cpf;
 
+  int run_pix = -1;
+  float run_v = 0.f;
   for (long long item = (long long)blockIdx.x * wpb + wave; item < A.n_items;
        item += (long long)gridDim.x * wpb) {
     const long long chunk = item % nchunks;
@@ -219,26 +221,48 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     const Sample smp = make_sample(A.S, pw, nw, A.weights[gi]);
     const int s0 = stile * A.source_tile;
     const int s1 = min(s0 + A.source_tile, A.ns);
+    const cptr<double> src = as_const(A.sources);
+    const cptr<float> vals = as_const(A.values);
+    double sx = src[3 * s0], sy = src[3 * s0 + 1], sz = src[3 * s0 + 2];
+    float sv = vals[s0];
     for (int s = s0; s < s1; ++s) {
+      // prefetch the next source while this one is traced
+      const int sn = s + 1 < s1 ? s + 1 : s;
+      const double nx = src[3 * sn], ny = src[3 * sn + 1], nz = src[3 * sn + 2];
+      const float nv = vals[sn];
       float x, y, v;
-      trace_ray<SRC, STAGES>(A, smp, active, A.sources[3 * s], A.sources[3 * s + 1],
-                             A.sources[3 * s + 2], A.values[s], x, y, v);
+      const bool ok = trace_ray<SRC, STAGES, AXIS>(A, smp, active, sx, sy, sz, sv, x, y, v);
+      sx = nx;
+      sy = ny;
+      sz = nz;
+      sv = nv;
       if (MODE == IACT_MODE_DEBUG) {
         if (active) {
           const size_t idx = ((size_t)f * A.ns + s) * A.M + m;
           A.out[idx] = v;
-          A.dbg_xy[2 * idx] = x;
-          A.dbg_xy[2 * idx + 1] = y;
-          if (A.dbg_pix) A.dbg_pix[idx] = hard_pixel<KIND>(A.S, table, x, y);
+          A.dbg_xy[2 * idx] = ok ? x : 1e10f;
+          A.dbg_xy[2 * idx + 1] = ok ? y : 1e10f;
+          if (A.dbg_pix) A.dbg_pix[idx] = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
         }
       } else if (!SPLAT) {
-        const int pix = active ? hard_pixel<KIND>(A.S, table, x, y) : -1;
-        if (A.lds_image)
+        const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        if (A.lds_image && A.acc == 1) {
+          // per-lane run of equal pixels, flushed when the pixel changes
+          if (pix != run_pix) {
+            if (run_pix >= 0 && run_v != 0.f) atomicAdd(img + run_pix, run_v);
+            run_pix = pix;
+            run_v = 0.f;
+          }
+          run_v += pix >= 0 ? v : 0.f;
+        } else if (A.lds_image && A.acc == 2) {
+          if (pix >= 0 && v != 0.f) atomicAdd(img + pix, v);
+        } else if (A.lds_image) {
           wave_accumulate<false>(img, pix, v, A.max_rounds);
-        else
+        } else {
           wave_accumulate<true>(A.out, pix, v, A.max_rounds);
+        }
       } else {
-        splat_taps<KIND>(A.S, table, x, y, active, [&](int pix, float w) {
+        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
           if (A.lds_image)
             wave_accumulate<false>(img, pix, v * w, A.max_rounds);
           else
@@ -249,6 +273,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   }
 
   if (MODE == IACT_MODE_IMAGE && A.lds_image) {
+    if (!SPLAT && A.acc == 1 && run_pix >= 0 && run_v != 0.f) atomicAdd(img + run_pix, run_v);
     __syncthreads();
     if (A.det) {
       for (int i = threadIdx.x; i < P; i += blockDim.x) {
@@ -279,7 +304,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
     for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
   }
   const int32_t* table = tab_lds;
-  const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
   const int lane = lane_id();
   const long long total = (long long)A.nf * A.M;
   const long long nchunks = (long long)A.nf * A.cpf;
@@ -299,15 +324,15 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
       const f3 n = mk3(A.world[3 * total + gi], A.world[4 * total + gi], A.world[5 * total + gi]);
       const Sample smp = make_sample(A.S, p, n, A.world[6 * total + gi]);
       float x, y, v;
-      trace_ray<SRC>(A, smp, active, sx, sy, sz, val, x, y, v);
+      const bool ok = trace_ray<SRC, true, false>(A, smp, active, sx, sy, sz, val, x, y, v);
       if (!SPLAT) {
-        const int pix = active ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
         if (A.lds_image)
           wave_accumulate<false>(row, pix, v, A.max_rounds);
         else
           wave_accumulate<true>(gdst, pix, v, A.max_rounds);
       } else {
-        splat_taps<KIND>(A.S, table, x, y, active, [&](int pix, float w) {
+        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
           if (A.lds_image)
             wave_accumulate<false>(row, pix, v * w, A.max_rounds);
           else
@@ -358,12 +383,20 @@ hipError_t launch_world(const iact_facets& F, const DevSensor& S, float* out, hi
 
 template <int SRC, int KIND, int SPLAT, int MODE>
 static hipError_t launch_trace_t(const TraceArgs& A, const LaunchShape& L, hipStream_t st) {
-  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0)
-    hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, false>), dim3(L.grid), dim3(L.threads),
-                       L.lds, st, A);
-  else
-    hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, true>), dim3(L.grid), dim3(L.threads),
-                       L.lds, st, A);
+  // image mode is specialised for primary-only scenes and identity sensor
+  // frames; both specialisations compute bit-identical results to the
+  // generic kernel, they only skip instructions
+  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0) {
+    if (A.S.axis)
+      hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, false, true>), dim3(L.grid),
+                         dim3(L.threads), L.lds, st, A);
+    else
+      hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, false, false>), dim3(L.grid),
+                         dim3(L.threads), L.lds, st, A);
+  } else {
+    hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, true, false>), dim3(L.grid),
+                       dim3(L.threads), L.lds, st, A);
+  }
   return hipGetLastError();
 }
 

@@ -46,25 +46,26 @@ struct TraceArgs {
   int lds_table;       // hex table staged in LDS
   int det;             // deterministic (wave-private images, ordered sums)
   int max_rounds;
+  int acc;             // 0 wave-merged, 1 per-lane run-length, 2 direct atomics
 };
 
 // One sample in world space, ready to be traced against many sources.
 struct Sample {
   f3 p, n;
-  float rw;        // 1 / weight
+  float nrw;       // -1 / weight (the reflected value is value * cos / w, cos = -d.n)
   double px, py, pz;
-  float ndoto;     // sensor normal . p (for the primary-only plane hit)
+  PlaneK K;        // sensor-plane constants of the primary origin
 };
 
 IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w) {
   Sample s;
   s.p = p;
   s.n = n;
-  s.rw = frcp(w);
+  s.nrw = -frcp(w);
   s.px = (double)p.x;
   s.py = (double)p.y;
   s.pz = (double)p.z;
-  s.ndoto = dot(p, S.nrm);
+  s.K = plane_consts(S, p);
   return s;
 }
 
@@ -153,7 +154,7 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
     }
   }
   const float c = dot(d, bn);
-  const f3 r = d - (2.f * c) * bn;
+  const f3 r = fma3(-2.f * c, bn, d);
   const float hit = best < 1e10f ? 1.f : 0.f;
   const float shadow = has_obstructions(A.ob) ? shadow_factor(A.ob, o, d, active) : 1.f;
   v = v * hit * shadow * fabsf(c);
@@ -162,30 +163,33 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
-template <int SRC, bool STAGES = true>
-IACT_DEV void trace_ray(const TraceArgs& A, const Sample& s, bool active, double sx, double sy,
+// Returns false when the ray misses the plane (the reference's 1e10
+// sentinel) or the lane is inactive; v is 0 for inactive lanes.
+template <int SRC, bool STAGES, bool AXIS>
+IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, double sx, double sy,
                         double sz, float value, float& x, float& y, float& v) {
   f3 d;
   if (SRC == IACT_SOURCE_POINT) {
     // p - src in float64 (sources sit at ~1e10 m), then normalised in fp32.
     const float ddx = (float)(s.px - sx), ddy = (float)(s.py - sy), ddz = (float)(s.pz - sz);
-    const float inv = frsq(ddx * ddx + ddy * ddy + ddz * ddz);
+    const float inv = frsq(fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz)));
     d = mk3(ddx * inv, ddy * inv, ddz * inv);
   } else {
     d = mk3((float)sx, (float)sy, (float)sz);  // used as given (Q15)
   }
-  const float shadow = has_obstructions(A.ob) ? shadow_factor(A.ob, s.p, -d, active) : 1.f;
   const float c = dot(d, s.n);
-  f3 r = d - (2.f * c) * s.n;
-  v = value * (-c) * s.rw * shadow;
-  f3 o = s.p;
-  float ndoto = s.ndoto;
+  f3 r = fma3(-2.f * c, s.n, d);
+  v = value * c * s.nrw;
+  if (has_obstructions(A.ob)) v *= shadow_factor(A.ob, s.p, -d, active);
+  PlaneK K = s.K;
   if (STAGES && A.n_stages > 0) {
+    f3 o = s.p;
     for (int k = 0; k < A.n_stages; ++k) reflect_at_stage(A, A.stages[k], o, r, v, active);
-    ndoto = dot(o, A.S.nrm);
+    K = plane_consts(A.S, o);
   }
-  plane_hit(A.S, o, r, ndoto, x, y);
+  const bool ok = plane_xy<AXIS>(A.S, K, r, x, y);
   if (!active) v = 0.f;
+  return ok && active;
 }
 
 template <int KIND>
@@ -196,53 +200,49 @@ IACT_DEV int hard_pixel(const DevSensor& S, const int32_t* table, float x, float
 
 // Splat taps of one ray (square.py:144-173, hexagonal.py:264-315).  Calls
 // fn(pixel, weight) for every tap; weights are normalised to sum 1.  Rays
-// with a non-finite coordinate or a zero normaliser contribute nothing.
+// that miss the plane or whose normaliser underflows contribute nothing.
 template <int KIND, typename Fn>
-IACT_DEV void splat_taps(const DevSensor& S, const int32_t* table, float x, float y, bool active,
+IACT_DEV void splat_taps(const DevSensor& S, const int32_t* table, float x, float y, bool ok_in,
                          Fn&& fn) {
+  const int ks = S.ks;
   if (KIND == IACT_SENSOR_SQUARE) {
-    const float xp = (x - S.x0) * S.inv_dx, yp = (y - S.y0) * S.inv_dy;
+    const float xp = fmaf(x, S.inv_dx, S.nx0), yp = fmaf(y, S.inv_dy, S.ny0);
     const float xb = floorf(xp), yb = floorf(yp);
     const float xf = xp - xb, yf = yp - yb;
-    const int ks = S.ks;
     float tot = 0.f;
     for (int oy = -ks; oy <= ks; ++oy)
       for (int ox = -ks; ox <= ks; ++ox) {
         const float ex = xf - (float)ox, ey = yf - (float)oy;
-        tot += __expf(S.neg_half_inv_sigma2 * (ex * ex + ey * ey));
+        tot += __expf(S.neg_half_inv_sigma2 * fmaf(ex, ex, ey * ey));
       }
-    const bool ok = active && tot > 0.f && fabsf(xb) < 1e8f && fabsf(yb) < 1e8f;
+    const bool ok = ok_in && tot > 0.f && fabsf(xb) < 1e8f && fabsf(yb) < 1e8f;
     const float it = ok ? frcp(tot) : 0.f;
     for (int oy = -ks; oy <= ks; ++oy)
       for (int ox = -ks; ox <= ks; ++ox) {
         const float ex = xf - (float)ox, ey = yf - (float)oy;
-        const float w = __expf(S.neg_half_inv_sigma2 * (ex * ex + ey * ey)) * it;
+        const float w = __expf(S.neg_half_inv_sigma2 * fmaf(ex, ex, ey * ey)) * it;
         const float xi = xb + (float)ox, yi = yb + (float)oy;
         const bool in = ok && xi >= 0.f && xi < S.wf && yi >= 0.f && yi < S.hf;
         fn(in ? (int)yi * S.width + (int)xi : -1, w);
       }
   } else {
-    float xg, yg, q, r, qb, rb;
-    hex_axial(S, x, y, xg, yg, q, r);
+    float q, r, qb, rb;
+    hex_axial(S, x, y, q, r);
     hex_round(q, r, qb, rb);
-    const float bx = S.size_s3 * (qb + rb * 0.5f), by = S.size_15 * rb;
-    const float ddx = xg - bx, ddy = yg - by;
-    const int ks = S.ks;
+    const float dq = q - qb, dr = r - rb;
     float tot = 0.f;
     for (int oq = -ks; oq <= ks; ++oq)
       for (int orr = -ks; orr <= ks; ++orr) {
         if (max(max(abs(oq), abs(orr)), abs(oq + orr)) > ks) continue;
-        const float ox = S.size_s3 * ((float)oq + (float)orr * 0.5f), oy = S.size_15 * (float)orr;
-        const float hd = hex_norm(S, ddx - ox, ddy - oy);
+        const float hd = hex_norm_axial(dq - (float)oq, dr - (float)orr);
         tot += __expf(S.neg_half_inv_sigma2 * hd * hd);
       }
-    const bool ok = active && tot > 0.f && fabsf(qb) < 1e8f && fabsf(rb) < 1e8f;
+    const bool ok = ok_in && tot > 0.f && fabsf(qb) < 1e8f && fabsf(rb) < 1e8f;
     const float it = ok ? frcp(tot) : 0.f;
     for (int oq = -ks; oq <= ks; ++oq)
       for (int orr = -ks; orr <= ks; ++orr) {
         if (max(max(abs(oq), abs(orr)), abs(oq + orr)) > ks) continue;
-        const float ox = S.size_s3 * ((float)oq + (float)orr * 0.5f), oy = S.size_15 * (float)orr;
-        const float hd = hex_norm(S, ddx - ox, ddy - oy);
+        const float hd = hex_norm_axial(dq - (float)oq, dr - (float)orr);
         const float w = __expf(S.neg_half_inv_sigma2 * hd * hd) * it;
         const int pid = ok ? hex_lookup(S, table, qb + (float)oq, rb + (float)orr) : -1;
         fn(pid, w);

@@ -30,11 +30,13 @@ __all__ = ["render", "render_debug", "render_response_matrix", "RenderPlan", "fa
 class LaunchConfig:
     """Launch shape handed to the kernels (see iactrace_amd.autotune)."""
 
-    def __init__(self, block_threads=0, source_tile=0, blocks_per_cu=0, deterministic=False):
+    def __init__(self, block_threads=0, source_tile=0, blocks_per_cu=0, deterministic=False,
+                 accumulate=0):
         self.block_threads = int(block_threads)
         self.source_tile = int(source_tile)
         self.blocks_per_cu = int(blocks_per_cu)
         self.deterministic = bool(deterministic)
+        self.accumulate = int(accumulate)
 
     def abi(self):
         L = _lib.IactLaunch()
@@ -42,12 +44,13 @@ class LaunchConfig:
         L.source_tile = self.source_tile
         L.blocks_per_cu = self.blocks_per_cu
         L.deterministic = int(self.deterministic)
+        L.accumulate = self.accumulate
         return L
 
     def __repr__(self):
         return (f"LaunchConfig(block_threads={self.block_threads}, source_tile="
                 f"{self.source_tile}, blocks_per_cu={self.blocks_per_cu}, "
-                f"deterministic={self.deterministic})")
+                f"deterministic={self.deterministic}, accumulate={self.accumulate})")
 
 
 def _stages(groups):

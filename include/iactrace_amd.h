@@ -144,6 +144,9 @@ typedef struct iact_launch {
   int32_t source_tile;     /* sources per work item                          */
   int32_t blocks_per_cu;   /* persistent grid = CUs * blocks_per_cu          */
   int32_t deterministic;   /* 1: fixed-order reductions, bitwise repeatable  */
+  int32_t accumulate;      /* pixel accumulation: 0 wave-merged (default),
+                              1 per-lane run-length, 2 direct per-ray atomics;
+                              deterministic mode always uses 0               */
 } iact_launch;
 
 /* Gradient outputs (fp64, overwritten).  NULL = not requested.               */
