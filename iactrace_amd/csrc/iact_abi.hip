@@ -29,11 +29,6 @@ int hip_error(hipError_t e, const char* where) {
 }
 
 // ------------------------------------------------------------- device info
-struct DevInfo {
-  int n_cu = 0;
-  int lds_per_block = 0;
-  char arch[64] = {0};
-};
 static std::mutex g_dev_mu;
 static DevInfo g_dev[64];
 static bool g_dev_ok[64];

@@ -5,7 +5,14 @@
 
 namespace iact {
 
-struct DevInfo;
+struct DevInfo {
+  int n_cu = 0;
+  int lds_per_block = 0;
+  char arch[64] = {0};
+};
+int device_info(DevInfo& out);
+int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, TraceArgs& A,
+                LaunchShape& out, const DevInfo& dev);
 int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* where);
 int validate_scene(const iact_scene* sc);

@@ -1,6 +1,398 @@
-// Adjoint (gradient) kernels -- placeholder entry points until the adjoint
-// kernel lands; they report IACT_ERR_UNSUPPORTED.
+// Adjoint (reverse-mode) kernel of the image render for primary-only scenes.
+//
+// The reference obtains gradients with jax.grad through render.py:174-220
+// (no custom VJP).  Here every ray is recomputed exactly as the forward
+// kernel traces it (same device functions, so the same pixel decisions) and
+// its adjoint is propagated by hand:
+//   image -> ray value (hard binning) or ray value + sensor-plane hit (splat)
+//   -> reflection -> source direction -> world sample (point, normal, weight).
+// A lane owns one MC sample and loops over a tile of sources; the adjoints of
+// the sample's world point, world normal and 1/weight are summed over the
+// sources in registers and pushed through the world transform (rotation
+// angles, position) and, when requested, the sample's surface (aspheric
+// coefficients) once per sample.  Source adjoints are wave-reduced per source.
+// Nothing per ray is stored.
+#include <math.h>
+#include <string.h>
+
 #include "iact_abi_internal.h"
+
+namespace iact {
+
+struct AdjArgs {
+  TraceArgs T;
+  const float* grad_image;
+  double* g_rot;
+  double* g_pos;
+  double* g_src;
+  double* g_val;
+  double* g_asph;
+  const int32_t* surface_index;
+  const float* offsets;
+};
+
+// dR/d(angle) of R = Rz(c) Ry(b) Rx(a) for each of the three angles (radians).
+IACT_DEV void euler_derivs(float tip, float tilt, float rot, m3& Da, m3& Db, m3& Dc) {
+  const float k = 0.017453292519943295f;
+  float sa, ca, sb, cb, sc, cc;
+  sincosf(tip * k, &sa, &ca);
+  sincosf(tilt * k, &sb, &cb);
+  sincosf(rot * k, &sc, &cc);
+  // Rz Ry = [[cc cb, -sc, cc sb], [sc cb, cc, sc sb], [-sb, 0, cb]]
+  // dRx/da = [[0,0,0],[0,-sa,-ca],[0,ca,-sa]]
+  const float A[9] = {cc * cb, -sc, cc * sb, sc * cb, cc, sc * sb, -sb, 0.f, cb};
+  for (int i = 0; i < 3; ++i) {
+    Da.m[3 * i + 0] = 0.f;
+    Da.m[3 * i + 1] = A[3 * i + 1] * (-sa) + A[3 * i + 2] * ca;
+    Da.m[3 * i + 2] = A[3 * i + 1] * (-ca) + A[3 * i + 2] * (-sa);
+  }
+  // dR/db = Rz dRy/db Rx, dRy/db = [[-sb,0,cb],[0,0,0],[-cb,0,-sb]]
+  // Rz dRy = [[-cc sb, 0, cc cb], [-sc sb, 0, sc cb], [-cb, 0, -sb]]
+  const float B[9] = {-cc * sb, 0.f, cc * cb, -sc * sb, 0.f, sc * cb, -cb, 0.f, -sb};
+  for (int i = 0; i < 3; ++i) {
+    Db.m[3 * i + 0] = B[3 * i + 0];
+    Db.m[3 * i + 1] = B[3 * i + 1] * ca + B[3 * i + 2] * sa;
+    Db.m[3 * i + 2] = -B[3 * i + 1] * sa + B[3 * i + 2] * ca;
+  }
+  // dR/dc = dRz/dc Ry Rx, dRz/dc Ry = [[-sc cb, -cc, -sc sb], [cc cb, -sc, cc sb], [0,0,0]]
+  const float C[9] = {-sc * cb, -cc, -sc * sb, cc * cb, -sc, cc * sb, 0.f, 0.f, 0.f};
+  for (int i = 0; i < 3; ++i) {
+    Dc.m[3 * i + 0] = C[3 * i + 0];
+    Dc.m[3 * i + 1] = C[3 * i + 1] * ca + C[3 * i + 2] * sa;
+    Dc.m[3 * i + 2] = -C[3 * i + 1] * sa + C[3 * i + 2] * ca;
+  }
+}
+
+IACT_DEV float frob(const m3& A, const float* B) {
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s = fmaf(A.m[i], B[i], s);
+  return s;
+}
+
+// Adjoint of the sensor-plane hit: given dL/dx, dL/dy, accumulate dL/do and
+// dL/dr for x = kx + t (r.u1), y = ky + t (r.u2), t = kt / (r.n).
+IACT_DEV void plane_adjoint(const DevSensor& S, f3 r, float t, float gx, float gy, f3& go,
+                            f3& gr) {
+  const float nd = dot(r, S.nrm);
+  const float ind = frcp(nd);
+  const float ru = dot(r, S.u1), rv = dot(r, S.u2);
+  // P1 = u1 - (r.u1 / r.n) n, P2 = u2 - (r.u2 / r.n) n
+  const f3 P1 = fma3(-ru * ind, S.nrm, S.u1);
+  const f3 P2 = fma3(-rv * ind, S.nrm, S.u2);
+  const f3 g = fma3(gx, P1, gy * P2);
+  go = go + g;
+  gr = fma3(t, g, gr);
+}
+
+// Splat tap weights and their gradient w.r.t. the sensor-plane hit:
+// returns sum_k G_k w_k and writes dL/dx, dL/dy for a ray of value v.
+template <int KIND>
+IACT_DEV float splat_adjoint(const DevSensor& S, const int32_t* table, const float* G, float x,
+                             float y, bool ok_in, float v, float& gx, float& gy) {
+  const int ks = S.ks;
+  const float h2 = 2.f * S.neg_half_inv_sigma2;
+  gx = 0.f;
+  gy = 0.f;
+  if (KIND == IACT_SENSOR_SQUARE) {
+    const float xp = fmaf(x, S.inv_dx, S.nx0), yp = fmaf(y, S.inv_dy, S.ny0);
+    const float xb = floorf(xp), yb = floorf(yp);
+    const float xf = xp - xb, yf = yp - yb;
+    float tot = 0.f, sx = 0.f, sy = 0.f;
+    for (int oy = -ks; oy <= ks; ++oy)
+      for (int ox = -ks; ox <= ks; ++ox) {
+        const float ex = xf - (float)ox, ey = yf - (float)oy;
+        const float e = __expf(S.neg_half_inv_sigma2 * fmaf(ex, ex, ey * ey));
+        tot += e;
+        sx = fmaf(e, ex, sx);
+        sy = fmaf(e, ey, sy);
+      }
+    const bool ok = ok_in && tot > 0.f && fabsf(xb) < 1e8f && fabsf(yb) < 1e8f;
+    if (!ok) return 0.f;
+    const float it = frcp(tot);
+    const float mx = sx * it, my = sy * it;
+    float gw = 0.f, ax = 0.f, ay = 0.f;
+    for (int oy = -ks; oy <= ks; ++oy)
+      for (int ox = -ks; ox <= ks; ++ox) {
+        const float ex = xf - (float)ox, ey = yf - (float)oy;
+        const float w = __expf(S.neg_half_inv_sigma2 * fmaf(ex, ex, ey * ey)) * it;
+        const float xi = xb + (float)ox, yi = yb + (float)oy;
+        const bool in = xi >= 0.f && xi < S.wf && yi >= 0.f && yi < S.hf;
+        const float g = in ? G[(int)yi * S.width + (int)xi] : 0.f;
+        gw = fmaf(g, w, gw);
+        ax = fmaf(g * w, ex - mx, ax);
+        ay = fmaf(g * w, ey - my, ay);
+      }
+    gx = v * h2 * ax * S.inv_dx;
+    gy = v * h2 * ay * S.inv_dy;
+    return gw;
+  } else {
+    float q, r, qb, rb;
+    hex_axial(S, x, y, q, r);
+    hex_round(q, r, qb, rb);
+    const float dq = q - qb, dr = r - rb;
+    float tot = 0.f, sq = 0.f, sr = 0.f;
+    // d ln e_k / d(dq, dr) = h2 hd_k (d hd_k / d(dq, dr))
+    for (int oq = -ks; oq <= ks; ++oq)
+      for (int orr = -ks; orr <= ks; ++orr) {
+        if (max(max(abs(oq), abs(orr)), abs(oq + orr)) > ks) continue;
+        const float a0 = dq - (float)oq, b0 = dr - (float)orr;
+        const float hd = hex_norm_axial(a0, b0);
+        const float e = __expf(S.neg_half_inv_sigma2 * hd * hd);
+        const float ee = fmaf(0.5f, b0, a0);
+        const float se = ee > 0.f ? 1.f : (ee < 0.f ? -1.f : 0.f);
+        const float sb = b0 > 0.f ? 1.f : (b0 < 0.f ? -1.f : 0.f);
+        const bool first = 2.f * fabsf(ee) >= fmaf(1.5f, fabsf(b0), fabsf(ee));
+        const float da = first ? 2.f * se : se;
+        const float db = first ? se : fmaf(0.5f, se, 1.5f * sb);
+        tot += e;
+        sq = fmaf(e, h2 * hd * da, sq);
+        sr = fmaf(e, h2 * hd * db, sr);
+      }
+    const bool ok = ok_in && tot > 0.f && fabsf(qb) < 1e8f && fabsf(rb) < 1e8f;
+    if (!ok) return 0.f;
+    const float it = frcp(tot);
+    const float mq = sq * it, mr = sr * it;
+    float gw = 0.f, aq = 0.f, ar = 0.f;
+    for (int oq = -ks; oq <= ks; ++oq)
+      for (int orr = -ks; orr <= ks; ++orr) {
+        if (max(max(abs(oq), abs(orr)), abs(oq + orr)) > ks) continue;
+        const float a0 = dq - (float)oq, b0 = dr - (float)orr;
+        const float hd = hex_norm_axial(a0, b0);
+        const float w = __expf(S.neg_half_inv_sigma2 * hd * hd) * it;
+        const float ee = fmaf(0.5f, b0, a0);
+        const float se = ee > 0.f ? 1.f : (ee < 0.f ? -1.f : 0.f);
+        const float sb = b0 > 0.f ? 1.f : (b0 < 0.f ? -1.f : 0.f);
+        const bool first = 2.f * fabsf(ee) >= fmaf(1.5f, fabsf(b0), fabsf(ee));
+        const float da = first ? 2.f * se : se;
+        const float db = first ? se : fmaf(0.5f, se, 1.5f * sb);
+        const int pid = hex_lookup(S, table, qb + (float)oq, rb + (float)orr);
+        const float g = pid >= 0 ? G[pid] : 0.f;
+        gw = fmaf(g, w, gw);
+        aq = fmaf(g * w, h2 * hd * da - mq, aq);
+        ar = fmaf(g * w, h2 * hd * db - mr, ar);
+      }
+    const float gq = v * aq, gr = v * ar;
+    gx = fmaf(gq, S.aqx, gr * S.arx);
+    gy = fmaf(gq, S.aqy, gr * S.ary);
+    return gw;
+  }
+}
+
+template <int SRC, int KIND, int SPLAT>
+__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
+  const TraceArgs& A = B.T;
+  extern __shared__ float smem[];
+  const int P = A.S.n_pixels;
+  float* G = smem;
+  int32_t* tab = (int32_t*)(smem + P);
+  for (int i = threadIdx.x; i < P; i += blockDim.x) G[i] = B.grad_image[i];
+  if (KIND == IACT_SENSOR_HEX)
+    for (int i = threadIdx.x; i < A.S.tq * A.S.tr; i += blockDim.x) tab[i] = A.S.table[i];
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+  const int lane = lane_id();
+  const long long nchunks = (long long)A.nf * A.cpf;
+  const bool want_src = B.g_src != nullptr, want_val = B.g_val != nullptr;
+  const bool want_asph = B.g_asph != nullptr && B.surface_index != nullptr && B.offsets != nullptr;
+
+  for (long long item = (long long)blockIdx.x * wpb + wave; item < A.n_items;
+       item += (long long)gridDim.x * wpb) {
+    const long long chunk = item % nchunks;
+    const int stile = (int)(item / nchunks);
+    const int f = (int)(chunk / A.cpf);
+    const int m = (int)(chunk % A.cpf) * 64 + lane;
+    const bool active = m < A.M;
+    const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
+    const m3 R = facet_rotation(A.rotations, f);
+    const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
+    const f3 pl = mk3(A.points[3 * gi], A.points[3 * gi + 1], A.points[3 * gi + 2]);
+    const f3 nl = mk3(A.normals[3 * gi], A.normals[3 * gi + 1], A.normals[3 * gi + 2]);
+    const f3 dl = mk3(A.delta[3 * gi], A.delta[3 * gi + 1], A.delta[3 * gi + 2]);
+    const float scale = A.pscale[f];
+    const float wgt = A.weights[gi];
+    f3 pw, nw;
+    world_sample(A.points, A.normals, A.delta, R, pos, scale, gi, pw, nw);
+    const Sample smp = make_sample(A.S, pw, nw, wgt);
+    f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
+    float Anrw = 0.f;
+    const int s0 = stile * A.source_tile;
+    const int s1 = min(s0 + A.source_tile, A.ns);
+    const cptr<double> src = as_const(A.sources);
+    const cptr<float> vals = as_const(A.values);
+    for (int s = s0; s < s1; ++s) {
+      const double sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+      const float value = vals[s];
+      // ---- forward recompute (the operations of trace_ray)
+      f3 d, u;
+      float inv = 1.f;
+      if (SRC == IACT_SOURCE_POINT) {
+        u = mk3((float)(smp.px - sx), (float)(smp.py - sy), (float)(smp.pz - sz));
+        inv = frsq(fmaf(u.x, u.x, fmaf(u.y, u.y, u.z * u.z)));
+        d = mk3(u.x * inv, u.y * inv, u.z * inv);
+      } else {
+        d = mk3((float)sx, (float)sy, (float)sz);
+      }
+      const float c = dot(d, smp.n);
+      const f3 r = fma3(-2.f * c, smp.n, d);
+      const float shadow =
+          has_obstructions(A.ob) ? shadow_factor(A.ob, smp.p, -d, active) : 1.f;
+      const float vunit = active ? c * smp.nrw * shadow : 0.f;  // ray value / source value
+      float x, y;
+      const bool ok = plane_xy<false>(A.S, smp.K, r, x, y) && active;
+      // ---- image adjoint -> ray adjoints
+      float gv, gx = 0.f, gy = 0.f;
+      if (!SPLAT) {
+        const int pix = ok ? hard_pixel<KIND>(A.S, tab, x, y) : -1;
+        gv = pix >= 0 ? G[pix] : 0.f;
+      } else {
+        gv = splat_adjoint<KIND>(A.S, tab, G, x, y, ok, value * vunit, gx, gy);
+      }
+      // value = value_s * c * nrw * shadow
+      const float dc = gv * value * smp.nrw * shadow;
+      Anrw = fmaf(gv * value * shadow, c, Anrw);
+      f3 gd = dc * smp.n;
+      An = fma3(dc, d, An);
+      if (SPLAT) {
+        // x, y depend on the origin (the sample) and on r = d - 2 (d.n) n
+        const float t = smp.K.kt * frcp(dot(r, A.S.nrm));
+        f3 gr = mk3(0.f, 0.f, 0.f);
+        plane_adjoint(A.S, r, t, gx, gy, Ap, gr);
+        const float ng = dot(smp.n, gr);
+        gd = gd + fma3(-2.f * ng, smp.n, gr);
+        An = An + fma3(-2.f * ng, d, (-2.f * c) * gr);
+      }
+      float gsx, gsy, gsz;
+      if (SRC == IACT_SOURCE_POINT) {
+        // d = u / |u|, u = p - src
+        const float dd = dot(d, gd);
+        const f3 gu = inv * fma3(-dd, d, gd);
+        Ap = Ap + gu;
+        gsx = -gu.x;
+        gsy = -gu.y;
+        gsz = -gu.z;
+      } else {
+        gsx = gd.x;
+        gsy = gd.y;
+        gsz = gd.z;
+      }
+      if (want_src || want_val) {
+        const float a = want_src ? wave_sum(active ? gsx : 0.f) : 0.f;
+        const float b = want_src ? wave_sum(active ? gsy : 0.f) : 0.f;
+        const float cz = want_src ? wave_sum(active ? gsz : 0.f) : 0.f;
+        const float e = want_val ? wave_sum(gv * vunit) : 0.f;
+        if (lane == 0) {
+          if (want_src) {
+            unsafeAtomicAdd(B.g_src + 3 * s, (double)a);
+            unsafeAtomicAdd(B.g_src + 3 * s + 1, (double)b);
+            unsafeAtomicAdd(B.g_src + 3 * s + 2, (double)cz);
+          }
+          if (want_val) unsafeAtomicAdd(B.g_val + s, (double)e);
+        }
+      }
+    }
+    if (!active) {
+      Ap = mk3(0.f, 0.f, 0.f);
+      An = mk3(0.f, 0.f, 0.f);
+      Anrw = 0.f;
+    }
+    // ---- per-sample adjoint through the world transform (mirrors.py:64-79)
+    // n = pert / |pert|, pert = R (nl + s dl);  p = R pl + pos
+    const f3 nls = fma3(scale, dl, nl);
+    const f3 pert = mul(R, nls);
+    const float ipn = frsq(dot(pert, pert));
+    const f3 gpert = ipn * fma3(-dot(nw, An), nw, An);
+    if (B.g_rot) {
+      float dR[9];
+      dR[0] = fmaf(Ap.x, pl.x, gpert.x * nls.x);
+      dR[1] = fmaf(Ap.x, pl.y, gpert.x * nls.y);
+      dR[2] = fmaf(Ap.x, pl.z, gpert.x * nls.z);
+      dR[3] = fmaf(Ap.y, pl.x, gpert.y * nls.x);
+      dR[4] = fmaf(Ap.y, pl.y, gpert.y * nls.y);
+      dR[5] = fmaf(Ap.y, pl.z, gpert.y * nls.z);
+      dR[6] = fmaf(Ap.z, pl.x, gpert.z * nls.x);
+      dR[7] = fmaf(Ap.z, pl.y, gpert.z * nls.y);
+      dR[8] = fmaf(Ap.z, pl.z, gpert.z * nls.z);
+      m3 Da, Db, Dc;
+      euler_derivs(A.rotations[3 * f], A.rotations[3 * f + 1], A.rotations[3 * f + 2], Da, Db,
+                   Dc);
+      const float k = 0.017453292519943295f;
+      const float ga = wave_sum(frob(Da, dR) * k);
+      const float gb = wave_sum(frob(Db, dR) * k);
+      const float gc = wave_sum(frob(Dc, dR) * k);
+      if (lane == 0) {
+        unsafeAtomicAdd(B.g_rot + 3 * f, (double)ga);
+        unsafeAtomicAdd(B.g_rot + 3 * f + 1, (double)gb);
+        unsafeAtomicAdd(B.g_rot + 3 * f + 2, (double)gc);
+      }
+    }
+    if (B.g_pos) {
+      const float a = wave_sum(Ap.x), b = wave_sum(Ap.y), cz = wave_sum(Ap.z);
+      if (lane == 0) {
+        unsafeAtomicAdd(B.g_pos + 3 * f, (double)a);
+        unsafeAtomicAdd(B.g_pos + 3 * f + 1, (double)b);
+        unsafeAtomicAdd(B.g_pos + 3 * f + 2, (double)cz);
+      }
+    }
+    if (want_asph) {
+      // the sample re-derived from its aperture position (x, y) on the
+      // surface: pl.z = sag(x+x0, y+y0) - sag(x0, y0), nl = normalize(-zx, -zy, 1),
+      // weight = nl.z * M / area (integrators.py:97-188); delta held fixed.
+      const int si = B.surface_index[f];
+      const iact_surface& su = A.surfaces[si];
+      const float x0 = B.offsets[2 * f], y0 = B.offsets[2 * f + 1];
+      const f3 gnl_w = mulT(R, gpert);   // dL/d nl (nl + s dl is rotated)
+      const f3 gpl = mulT(R, Ap);        // dL/d pl
+      // weight: nrw = -1/w, dL/dw = Anrw / w^2; w = nl.z * const
+      const float dLdw = Anrw * frcp(wgt * wgt);
+      f3 gnl = gnl_w;
+      gnl.z = fmaf(dLdw, wgt * frcp(nl.z), gnl.z);
+      // nl = mvec / |mvec|, mvec = (-zx, -zy, 1), |mvec| = 1 / nl.z
+      const f3 gm = nl.z * fma3(-dot(nl, gnl), nl, gnl);
+      const float X = pl.x + x0, Y = pl.y + y0;
+      const float r2 = fmaf(X, X, Y * Y), r20 = fmaf(x0, x0, y0 * y0);
+      const float r4 = r2 * r2, r40 = r20 * r20;
+      float pw_ = r4, pw0 = r40, q = 1.f;
+      for (int i = 0; i < IACT_MAX_ASPHERIC; ++i) {
+        if (i >= su.n_aspheric) break;
+        const float dz = pw_ - pw0;                         // d pl.z / d a_i
+        const float dg = (float)(2 * (i + 1)) * r2 * q;     // d (dz/dr2) / d a_i
+        const float dzx = 2.f * X * dg, dzy = 2.f * Y * dg;  // d (zx, zy) / d a_i
+        const float contrib = active ? fmaf(gpl.z, dz, -(gm.x * dzx + gm.y * dzy)) : 0.f;
+        const float tot = wave_sum(contrib);
+        if (lane == 0) unsafeAtomicAdd(B.g_asph + IACT_MAX_ASPHERIC * si + i, (double)tot);
+        pw_ *= r4;
+        pw0 *= r40;
+        q *= r4;
+      }
+    }
+  }
+}
+
+template <int SRC, int KIND, int SPLAT>
+static hipError_t launch_adj_t(const AdjArgs& B, const LaunchShape& L, hipStream_t st) {
+  hipLaunchKernelGGL((k_trace_adj<SRC, KIND, SPLAT>), dim3(L.grid), dim3(L.threads), L.lds, st,
+                     B);
+  return hipGetLastError();
+}
+
+template <int SRC>
+static hipError_t dispatch_adj(const AdjArgs& B, const LaunchShape& L, hipStream_t st) {
+  const DevSensor& S = B.T.S;
+  if (S.kind == IACT_SENSOR_SQUARE)
+    return S.splat ? launch_adj_t<SRC, IACT_SENSOR_SQUARE, 1>(B, L, st)
+                   : launch_adj_t<SRC, IACT_SENSOR_SQUARE, 0>(B, L, st);
+  return S.splat ? launch_adj_t<SRC, IACT_SENSOR_HEX, 1>(B, L, st)
+                 : launch_adj_t<SRC, IACT_SENSOR_HEX, 0>(B, L, st);
+}
+
+hipError_t launch_adjoint(const AdjArgs& B, const LaunchShape& L, int source_type,
+                          hipStream_t st) {
+  return source_type == IACT_SOURCE_POINT ? dispatch_adj<IACT_SOURCE_POINT>(B, L, st)
+                                          : dispatch_adj<IACT_SOURCE_PARALLEL>(B, L, st);
+}
+
+}  // namespace iact
 
 using namespace iact;
 
@@ -8,35 +400,68 @@ extern "C" {
 
 size_t iact_bwd_workspace_size(const iact_scene*, int32_t, const iact_launch*) { return 0; }
 
-int32_t iact_trace_bwd(const iact_scene*, const double*, const float*, int32_t, int32_t,
-                       const float*, const iact_launch*, void*, size_t, const iact_grads*,
-                       void*) {
-  return set_error(IACT_ERR_UNSUPPORTED, "iact_trace_bwd not built yet");
+int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const float* values,
+                       int32_t n_sources, int32_t source_type, const float* grad_image,
+                       const iact_launch* launch, void* workspace, size_t workspace_bytes,
+                       const iact_grads* grads, void* stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  int rc = validate_scene(scene);
+  if (rc) return rc;
+  if (!grads) return set_error(IACT_ERR_INVALID, "null gradient descriptor");
+  if (n_sources < 0) return set_error(IACT_ERR_INVALID, "negative source count");
+  if (n_sources > 0 && (!sources || !values))
+    return set_error(IACT_ERR_INVALID, "null sources/values");
+  if (!grad_image) return set_error(IACT_ERR_INVALID, "null image gradient");
+  if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL)
+    return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
+  const iact_scene& sc = *scene;
+  if (sc.n_stages > 0)
+    return set_error(IACT_ERR_UNSUPPORTED,
+                     "the adjoint kernel covers primary-only scenes; stage>=1 mirrors need the "
+                     "tangent kernel");
+  if (grads->aspheric && (!sc.facets.surface_index || !sc.facets.offsets))
+    return set_error(IACT_ERR_INVALID, "aspheric gradients need facet surface_index/offsets");
+  hipStream_t st = (hipStream_t)stream;
+  const int nf = sc.facets.n_facets;
+  hipError_t e;
+  if (grads->rotations && (e = launch_fill_d(grads->rotations, 3ll * nf, 0.0, st)) != hipSuccess)
+    return hip_error(e, "clear grad");
+  if (grads->positions && (e = launch_fill_d(grads->positions, 3ll * nf, 0.0, st)) != hipSuccess)
+    return hip_error(e, "clear grad");
+  if (grads->sources &&
+      (e = launch_fill_d(grads->sources, 3ll * n_sources, 0.0, st)) != hipSuccess)
+    return hip_error(e, "clear grad");
+  if (grads->values && (e = launch_fill_d(grads->values, n_sources, 0.0, st)) != hipSuccess)
+    return hip_error(e, "clear grad");
+  if (grads->aspheric &&
+      (e = launch_fill_d(grads->aspheric, (long long)IACT_MAX_SURFACES * IACT_MAX_ASPHERIC, 0.0,
+                         st)) != hipSuccess)
+    return hip_error(e, "clear grad");
+  if (nf == 0 || n_sources == 0) return IACT_OK;
+  DevInfo dev;
+  rc = device_info(dev);
+  if (rc) return rc;
+  AdjArgs B;
+  memset(&B, 0, sizeof(B));
+  fill_trace_args(B.T, sc, sources, values, n_sources);
+  LaunchShape shp;
+  rc = shape_trace(sc, n_sources, IACT_MODE_DEBUG, launch, B.T, shp, dev);
+  if (rc) return rc;
+  const size_t lds = (size_t)sc.sensor.n_pixels * sizeof(float) + shp.lds;
+  if (lds > 65536)
+    return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS", lds);
+  shp.lds = lds;
+  B.grad_image = grad_image;
+  B.g_rot = grads->rotations;
+  B.g_pos = grads->positions;
+  B.g_src = grads->sources;
+  B.g_val = grads->values;
+  B.g_asph = grads->aspheric;
+  B.surface_index = sc.facets.surface_index;
+  B.offsets = sc.facets.offsets;
+  e = launch_adjoint(B, shp, source_type, st);
+  return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_adj");
 }
 
-size_t iact_batch_workspace_size(const iact_scene*, const iact_batch*, const iact_launch*) {
-  return 0;
-}
-int32_t iact_trace_batched_fwd(const iact_scene*, const iact_batch*, int32_t, const iact_launch*,
-                               void*, size_t, float*, void*) {
-  return set_error(IACT_ERR_UNSUPPORTED, "batched trace not built yet");
-}
-int32_t iact_trace_batched_bwd(const iact_scene*, const iact_batch*, int32_t, const float*,
-                               const iact_launch*, void*, size_t, double*, void*) {
-  return set_error(IACT_ERR_UNSUPPORTED, "batched trace not built yet");
-}
-int32_t iact_l2_loss_grad(const float*, const float*, int64_t, int32_t, float*, double*, void*) {
-  return set_error(IACT_ERR_UNSUPPORTED, "not built yet");
-}
-int32_t iact_adam_step(float*, const double*, float*, float*, int32_t*, int64_t, float, float,
-                       float, float, const int32_t*, void*) {
-  return set_error(IACT_ERR_UNSUPPORTED, "not built yet");
-}
-int32_t iact_fit_graph_create(const iact_scene*, const iact_batch*, int32_t, const iact_launch*,
-                              void*, size_t, float*, const float*, float*, double*, double*,
-                              float*, float*, float*, int32_t*, float, void**) {
-  return set_error(IACT_ERR_UNSUPPORTED, "not built yet");
-}
-int32_t iact_graph_launch(void*, void*) { return set_error(IACT_ERR_UNSUPPORTED, "not built yet"); }
-int32_t iact_graph_destroy(void*) { return set_error(IACT_ERR_UNSUPPORTED, "not built yet"); }
-}
+}  // extern "C"

@@ -430,16 +430,14 @@ IACT_DEV void hex_axial(const DevSensor& S, float x, float y, float& q, float& r
 }
 
 // Cube rounding, half-to-even (hexagonal.py:32-39).
-IACT_DEV void hex_round(float q, float r, float& qi, float& ri) {
+IACT_DEV void hex_round(float q, float r, float& qi_out, float& ri_out) {
   const float s = -q - r;
-  qi = rintf(q);
-  ri = rintf(r);
-  const float si = rintf(s);
+  const float qi = rintf(q), ri = rintf(r), si = rintf(s);
   const float dq = fabsf(qi - q), dr = fabsf(ri - r), ds = fabsf(si - s);
-  if (dq > dr && dq > ds)
-    qi = -ri - si;
-  else if (dr > dq && dr > ds)
-    ri = -qi - si;
+  const bool fix_q = dq > dr && dq > ds;
+  const bool fix_r = !fix_q && dr > dq && dr > ds;  // the q fix-up cannot also hold
+  qi_out = fix_q ? -ri - si : qi;
+  ri_out = fix_r ? -qi - si : ri;
 }
 
 // Table lookup with bounds (hexagonal.py:155-172); -1 when off-table/empty.

@@ -305,10 +305,12 @@ class _TraceFunction(torch.autograd.Function):
         need = (nig[2], nig[3], nig[4], nig[5], any(nig[6:]))
         outs = ctx.plan.backward(grad, sources, values, ctx.source_type, rotations, positions,
                                  need)
-        g_src = outs["sources"].to(sources.dtype).reshape(sources.shape) if nig[2] else None
-        g_val = outs["values"].to(values.dtype).reshape(values.shape) if nig[3] else None
-        g_rot = outs["rotations"].to(rotations.dtype) if nig[4] else None
-        g_pos = outs["positions"].to(positions.dtype) if nig[5] else None
+        def _like(g, ref):
+            return g.to(device=ref.device, dtype=ref.dtype).reshape(ref.shape)
+        g_src = _like(outs["sources"], sources) if nig[2] else None
+        g_val = _like(outs["values"], values) if nig[3] else None
+        g_rot = _like(outs["rotations"], rotations) if nig[4] else None
+        g_pos = _like(outs["positions"], positions) if nig[5] else None
         g_asph = []
         for i in range(ctx.n_asph):
             if nig[6 + i]:

@@ -109,3 +109,26 @@ def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
         _, _, v2, _ = O.render(sc2, sources, values, source_type, "fp64", "debug")
         band |= (v2 != 0) != alive
     return band, x, y, v
+
+
+def hex_kink_margin(x, y, sensor):
+    """Smallest distance (axial units) of each ray from a kink of the hex-norm
+    splat weight of any of its taps: the norm max(2|e|, |e| + 1.5|b|) of the
+    tap offset (e = dq + db/2, b = dr) has a discontinuous derivative where
+    |e| = 1.5|b|, e = 0 or b = 0, so fp32 and fp64 may take different
+    branches there and their gradients differ by O(1) for that ray."""
+    xg, yg = O.hex_grid_coords(np.asarray(x, np.float64), np.asarray(y, np.float64), sensor)
+    q = (O.SQRT3_3 * xg - yg / 3) / sensor.hex_size
+    r = (2 * yg / 3) / sensor.hex_size
+    qb, rb = O.axial_round(q, r)
+    dq, dr = q - qb, r - rb
+    ks = sensor.kernel_size
+    m = np.full(np.shape(xg), np.inf)
+    for a in range(-ks, ks + 1):
+        for b in range(-ks, ks + 1):
+            if max(abs(a), abs(b), abs(a + b)) > ks:
+                continue
+            e = np.abs(dq - a + 0.5 * (dr - b))
+            bb = np.abs(dr - b)
+            m = np.minimum(m, np.minimum(np.abs(e - 1.5 * bb), np.minimum(e, bb)))
+    return m

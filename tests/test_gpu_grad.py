@@ -1,0 +1,236 @@
+"""Gradients of the HIP adjoint kernel (GPU).
+
+* against torch float64 autograd of the oracle twin (oracle/torch_twin.py),
+  which stands in for jax.grad of the reference: rotations, positions,
+  sources, values and primary aspheric coefficients, for hard and
+  Gaussian-splat sensors, point and parallel sources, with obstructions;
+* against central finite differences of the GPU forward render
+  (h = 1e-4 rad) on splat sensors and on the total flux of a hard sensor.
+Losses weight the image with a smooth pixel map so that rays in the binning
+edge band (assigned to a neighbouring pixel in fp32 vs fp64) barely matter.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import iactrace_amd as ia
+import oracle.torch_twin as TW
+from iactrace_amd import configs
+from tests.helpers import band_mask, hex_kink_margin, oracle_scene
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _smooth_weights(sensor):
+    if isinstance(sensor, ia.SquareSensor):
+        yy, xx = np.mgrid[0:sensor.height, 0:sensor.width]
+        return (1.0 + 0.3 * np.sin(xx / sensor.width * 3.0) + 0.2 * yy / sensor.height).reshape(-1)
+    c = np.asarray(sensor.hex_centers, np.float64)
+    return 1.0 + 0.3 * np.sin(c[:, 0] * 3.0) + 0.2 * c[:, 1]
+
+
+def _kernel_grads(tel, src, vals, stype, W, asph=False):
+    g = tel.mirror_groups[0]
+    rot = g.rotations.detach().clone().requires_grad_(True)
+    pos = g.positions.detach().clone().requires_grad_(True)
+    groups = list(tel.mirror_groups)
+    a0 = g.aspheric.detach().clone().requires_grad_(asph)
+    groups[0] = g.replace(rotations=rot, positions=pos, aspheric=a0)
+    t2 = ia.Telescope(groups, tel.obstruction_groups, tel.sensors)
+    s = torch.tensor(src, dtype=torch.float64, device=DEV, requires_grad=True)
+    v = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+    img = ia.render(t2, s, v, stype)
+    loss = (img.reshape(-1).double() * torch.as_tensor(W, device=DEV)).sum()
+    loss.backward()
+    out = {"rot": rot.grad.double().cpu().numpy(), "pos": pos.grad.double().cpu().numpy(),
+           "src": s.grad.double().cpu().numpy(), "val": v.grad.double().cpu().numpy()}
+    if asph:
+        out["asph"] = a0.grad.double().cpu().numpy()
+    return out, float(loss)
+
+
+def _twin_grads(tel, src, vals, stype, W, asph=False):
+    sc = oracle_scene(tel)
+    g = tel.mirror_groups[0]
+    rot = torch.tensor(sc.rotations, dtype=torch.float64, requires_grad=True)
+    pos = torch.tensor(sc.positions, dtype=torch.float64, requires_grad=True)
+    s = torch.tensor(src, dtype=torch.float64, requires_grad=True)
+    v = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    kw = {}
+    if asph:
+        kw = dict(aspheric0=torch.tensor(g.aspheric.double().numpy(), requires_grad=True),
+                  surface0=(g.curvature, g.conic), offsets0=g.offsets.cpu().numpy())
+    img = TW.render_image(sc, s, v, stype, rot, pos, **kw)
+    loss = (img * torch.as_tensor(W)).sum()
+    loss.backward()
+    z = lambda t: (t.grad if t.grad is not None else torch.zeros_like(t)).numpy()  # noqa: E731
+    out = {"rot": z(rot), "pos": z(pos), "src": z(s), "val": z(v)}
+    if asph:
+        out["asph"] = z(kw["aspheric0"])
+    return out, float(loss)
+
+
+def _compare(label, gk, gt, keys, tol=1e-3, facet_mask=None, zero_keys=()):
+    for k in keys:
+        a, b = gk[k], gt[k]
+        if facet_mask is not None and k in ("rot", "pos"):
+            a, b = a[facet_mask], b[facet_mask]
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / max(scale, 1e-300)
+        print(f"[{label}] d/d{k}: max|kernel - twin| / max|twin| = {err:.2e} (scale {scale:.3g})",
+              flush=True)
+        if k in zero_keys:
+            assert scale == 0 and np.abs(a).max() == 0, k
+            continue
+        assert scale > 0, k
+        assert err < tol, (k, err)
+
+
+def test_grad_hard_square_point_sources():
+    wl = configs.workload("C2", n_sources=6, n_samples=16)
+    tel = wl.telescope(DEV, key=3)
+    W = _smooth_weights(tel.sensors[0])
+    gk, lk = _kernel_grads(tel, wl.sources, wl.values, "point", W)
+    gt, lt = _twin_grads(tel, wl.sources, wl.values, "point", W)
+    assert abs(lk - lt) < 1e-4 * abs(lt)
+    _compare("hard-square", gk, gt, ["rot", "pos", "src", "val"])
+
+
+def test_grad_splat_hex_obstructions():
+    wl = configs.workload("C3", n_sources=3, n_samples=8)
+    tel = wl.telescope(DEV, key=5)
+    s = tel.sensors[0]
+    tel = tel.replace_sensor(ia.DifferentiableHexagonalSensor(s.position, s.rotation,
+                                                              s.hex_centers, 0.5, 1), 0)
+    src, vals = configs.point_source_field(3, 1.2, seed=9)
+    W = _smooth_weights(tel.sensors[0])
+    gk, _ = _kernel_grads(tel, src, vals, "point", W)
+    gt, _ = _twin_grads(tel, src, vals, "point", W)
+    # a facet with a ray whose nearest hexagon (which fixes the splat's tap
+    # set), shadow decision or hex-norm branch (a kink of the tap weight) can
+    # flip between fp32 and fp64 carries an O(1/rays) difference; those facets
+    # are left out of the pose comparison
+    sc = oracle_scene(tel)
+    band, x, y, _ = band_mask(sc, src, vals, "point")
+    band |= hex_kink_margin(x, y, sc.sensor) < 2e-4
+    nf = sc.points.shape[0]
+    keep = ~band.reshape(nf, len(src), -1).any((1, 2))
+    print(f"[splat-hex] facets compared {int(keep.sum())} of {nf}", flush=True)
+    _compare("splat-hex", gk, gt, ["rot", "pos", "src", "val"], facet_mask=keep)
+
+
+def test_grad_splat_square_parallel():
+    wl = configs.workload("C1", n_samples=16)
+    tel = wl.telescope(DEV, key=2)
+    s = tel.sensors[0]
+    tel = tel.replace_sensor(ia.DifferentiableSquareSensor(s.position, s.rotation, s.width,
+                                                           s.height, s.bounds, 0.4, 2), 0)
+    src = configs.parallel_grid(3, 2, 0.5)
+    vals = np.array([1.0, 0.5, 2.0, 1.5, 0.7, 1.1], np.float32)
+    W = _smooth_weights(tel.sensors[0])
+    gk, _ = _kernel_grads(tel, src, vals, "parallel", W)
+    gt, _ = _twin_grads(tel, src, vals, "parallel", W)
+    _compare("splat-square-parallel", gk, gt, ["rot", "pos", "src", "val"])
+
+
+def _aspheric_primary_tel(m=512, splat=False):
+    mirrors, _ = configs.cassegrain()
+    prim = mirrors[0]
+    prim.surface.aspheric = torch.tensor([3e-4, -2e-6, 1e-8, -5e-11])
+    if splat:
+        sensor = ia.DifferentiableSquareSensor([0, 0, 9.5], [0, 0, 0], 32, 32,
+                                               (-0.4, 0.4, -0.4, 0.4), sigma=0.6, kernel_size=2)
+    else:
+        sensor = ia.SquareSensor([0, 0, 9.5], [0, 0, 0], 32, 32, (-0.4, 0.4, -0.4, 0.4))
+    g = ia.group_mirrors([prim])[0].to(DEV)
+    g = ia.MCIntegrator(m).sample_group(g, 11)
+    return ia.Telescope([g], [], [sensor])
+
+
+@pytest.mark.parametrize("splat", [False, True])
+def test_grad_primary_aspheric(splat):
+    tel = _aspheric_primary_tel(splat=splat)
+    src = configs.parallel_grid(2, 2, 0.3)
+    vals = np.ones(4, np.float32)
+    W = _smooth_weights(tel.sensors[0])
+    gk, _ = _kernel_grads(tel, src, vals, "parallel", W, asph=True)
+    gt, _ = _twin_grads(tel, src, vals, "parallel", W, asph=True)
+    # parallel rays on a hard sensor: moving a facet only moves its spot
+    # across pixel boundaries, so the position gradient is exactly zero
+    _compare(f"aspheric-{'splat' if splat else 'hard'}", gk, gt, ["asph", "rot", "pos"],
+             tol=2e-3, zero_keys=() if splat else ("pos",))
+
+
+def _facet(tel, f):
+    """Telescope holding only facet f of the primary group (its rays are the
+    only ones that depend on its pose)."""
+    g = tel.mirror_groups[0]
+    kw = {}
+    for name in ("positions", "rotations", "points", "normals", "weights", "perturbation_delta",
+                 "perturbation_scale", "offsets", "radii", "vertices"):
+        t = getattr(g, name, None)
+        if isinstance(t, torch.Tensor):
+            kw[name] = t[f:f + 1].detach().clone()
+    return ia.Telescope([g.replace(**kw)], tel.obstruction_groups, tel.sensors)
+
+
+DET = ia.LaunchConfig(deterministic=True)
+
+
+def _loss_fd(tel1, src, vals, stype, W, rot):
+    t2 = tel1.set_mirror_rotations(0, rot)
+    img = ia.render(t2, torch.as_tensor(src), torch.as_tensor(vals), stype, launch=DET)
+    return float((img.reshape(-1).double() * torch.as_tensor(W, device=DEV)).sum())
+
+
+def _fd_check(label, tel, src, vals, stype, W, facets, h_rad, rtol):
+    """Richardson-extrapolated central differences of single-facet
+    deterministic renders (error O(h^4)) against the adjoint of the full
+    render."""
+    gk, _ = _kernel_grads(tel, src, vals, stype, W)
+    scale = np.abs(gk["rot"]).max()
+    for f in facets:
+        t1 = _facet(tel, int(f))
+        rot0 = t1.mirror_groups[0].rotations.detach().clone()
+        for k in (0, 1):
+            def cd(h):
+                rp, rm = rot0.clone(), rot0.clone()
+                rp[0, k] += h
+                rm[0, k] -= h
+                return (_loss_fd(t1, src, vals, stype, W, rp) -
+                        _loss_fd(t1, src, vals, stype, W, rm)) / (2 * h)
+            h = math.degrees(h_rad)
+            fd = (4 * cd(h / 2) - cd(h)) / 3
+            ad = gk["rot"][f, k]
+            print(f"[{label}] facet {f} angle {k}: adjoint {ad:.7g} fd {fd:.7g}", flush=True)
+            assert abs(fd - ad) <= rtol * max(abs(fd), 0.05 * scale)
+
+
+def test_grad_vs_central_differences_splat():
+    """Square Gaussian splat (sigma 0.7 px, 7x7 taps: sampling ripple and tap
+    truncation both below 1e-4) on the Davies-Cotton dish; no obstructions,
+    whose shadow edges would make the difference quotient jump."""
+    wl = configs.workload("C2", n_sources=2, n_samples=256)
+    tel = wl.telescope(DEV, key=7)
+    sensor = ia.DifferentiableSquareSensor([0, 0, 15.0], [0, 0, 0], 64, 64,
+                                           (-0.8, 0.8, -0.8, 0.8), sigma=0.7, kernel_size=3)
+    tel = tel.replace_sensor(sensor, 0)
+    src, vals = configs.point_source_field(2, 0.5, seed=1)
+    yy, xx = np.mgrid[0:64, 0:64]
+    W = (1.0 + 0.5 * np.sin(xx / 64 * 5.0) + 0.4 * np.cos(yy / 64 * 4.0)).reshape(-1)
+    _fd_check("fd-splat", tel, src, vals, "point", W, [0, 41, 190, 300, 377], 1e-4, 2e-3)
+
+
+def test_grad_vs_central_differences_total_flux_hard():
+    """Total flux on a hard sensor that catches every ray: only the facet
+    weight depends on the tilt, a relative change of ~1e-3 per mrad, so the
+    step is large (1e-2 rad) and Richardson removes its O(h^2) error."""
+    wl = configs.workload("C2", n_sources=4, n_samples=1024)
+    tel = wl.telescope(DEV, key=8)
+    big = ia.SquareSensor([0, 0, 15.0], [0, 0, 0], 16, 16, (-2.0, 2.0, -2.0, 2.0))
+    tel = tel.replace_sensor(big, 0)
+    _fd_check("fd-flux", tel, wl.sources, wl.values, "point", np.ones(big.n_pixels),
+              [0, 17, 200, 381], 1e-2, 2e-3)
