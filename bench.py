@@ -55,11 +55,52 @@ def time_steps(fn, steps, warmup):
 
 
 def traffic_from_profiles(name):
-    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    """HBM bytes per k_trace launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", f"k_trace_{name}_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
+
+
+def c3_fwd_bwd(ia, configs, steps, warmup):
+    """C3 (875 facets x 1024 samples x 16384 point sources, hex camera, box +
+    4 cylinders): forward rays/s and forward+backward rays/s, the backward
+    being the gradient of a weighted image sum wrt all facet tilts."""
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    wl = configs.workload("C3")
+    tel = wl.telescope(dev, key=0)
+    src = torch.as_tensor(wl.sources, dtype=torch.float64, device=dev)
+    vals = torch.as_tensor(wl.values, dtype=torch.float32, device=dev)
+    W = torch.linspace(0.5, 1.5, tel.sensors[0].n_pixels, device=dev)
+    g = tel.mirror_groups[0]
+    rot = g.rotations.detach().clone().requires_grad_(True)
+    tel_g = ia.Telescope([g.replace(rotations=rot)] + list(tel.mirror_groups[1:]),
+                         tel.obstruction_groups, tel.sensors)
+
+    def fwd():
+        return ia.render(tel, src, vals, wl.source_type)
+
+    def fwd_bwd():
+        rot.grad = None
+        loss = (ia.render(tel_g, src, vals, wl.source_type) * W).sum()
+        loss.backward()
+
+    wf, _ = time_steps(fwd, steps, warmup)
+    wb, _ = time_steps(fwd_bwd, steps, warmup)
+    return {"workload": f"C3: parabolic f=36 m, {wl.n_facets} facets x {wl.n_samples} samples "
+                        f"x {len(wl.sources)} point sources, hex 2048 px, box + 4 cylinders, "
+                        "tilt errors 0.5 mrad",
+            "rays_per_step": wl.n_rays, "steps": steps,
+            "fwd_rays_per_s": wl.n_rays * steps / wf, "fwd_ms": 1e3 * wf / steps,
+            "fwd_bwd_rays_per_s": wl.n_rays * steps / wb, "fwd_bwd_ms": 1e3 * wb / steps,
+            "bwd_grad": "d(sum(W*image))/d(facet tilts)"}
 
 
 def cpu_baseline(wl, tel, target_s=15.0):
@@ -97,6 +138,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the C3 forward+backward line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (for counter calibration)")
     args = ap.parse_args()
@@ -174,6 +217,8 @@ def main():
         b.copy_(a)
         _sync()
         out["copy_calibration_bytes"] = {"fetch": a.numel() * 4, "write": b.numel() * 4}
+    if not args.no_extras and args.config == "C2":
+        out["extras"] = {"C3_fwd_bwd": c3_fwd_bwd(ia, configs, 3, 1)}
     if not args.no_cpu:
         cb, cimg, n = cpu_baseline(wl, tel)
         out["cpu_baseline"] = cb
