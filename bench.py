@@ -131,6 +131,55 @@ def cpu_baseline(wl, tel, target_s=15.0):
                       f"with fp64 directions, gcc -O3 -march=x86-64-v3 -fopenmp"}, img, n
 
 
+def c5_batched(ia, configs, n_instances):
+    """C5: instances of the C3 dish with their own tilt draws and uneven
+    source lists (U{256..16384}).  Times (a) the batched forward, (b) one
+    alignment-fit iteration -- batched forward, L2 loss vs the untilted
+    render, batched adjoint, Adam -- replayed as a HIP graph, and (c) the
+    same forward+backward done plainly, one render + autograd per instance."""
+    import torch
+    from iactrace_amd.batched import AlignmentFit, BatchedPlan
+    dev = torch.device("cuda:0")
+    bw = configs.batch_workload(n_instances)
+    tel = bw.telescope(dev, key=0)
+    rot = bw.rotations(tel)
+    base = tel.mirror_groups[0].rotations.detach()
+    bp = BatchedPlan(tel, bw.sources, bw.values, bw.offsets, "point")
+    with torch.no_grad():
+        targets = bp.forward(base.unsqueeze(0).repeat(n_instances, 1, 1).contiguous())
+    wf, _ = time_steps(lambda: bp.forward(rot), 1, 0)
+    fit = AlignmentFit(tel, rot, bw.sources, bw.values, bw.offsets, targets, lr=1e-4)
+    wg, _ = time_steps(fit.step, 1, 1)
+    loss = float(fit.loss.sum())
+    fit.close()
+    g = tel.mirror_groups[0]
+    src = torch.as_tensor(bw.sources, device=dev)
+    val = torch.as_tensor(bw.values, device=dev)
+    tgt = targets
+
+    def plain():
+        for i in range(n_instances):
+            a, b = int(bw.offsets[i]), int(bw.offsets[i + 1])
+            r = rot[i].clone().requires_grad_(True)
+            t = ia.Telescope([g.replace(rotations=r)], tel.obstruction_groups, tel.sensors)
+            img = ia.render(t, src[a:b], val[a:b], "point").reshape(-1)
+            ((img - tgt[i]) ** 2).sum().backward()
+
+    wp, _ = time_steps(plain, 1, 0)
+    n = bw.n_rays
+    return {"workload": f"C5: {n_instances} x C3 dish (875 facets x 1024 samples, hex 2048 px, "
+                        f"box + 4 cylinders), independent 0.5 mrad tilt draws, "
+                        f"{int(bw.offsets[-1])} point sources in total (U{{256..16384}} per "
+                        "instance)",
+            "rays_per_iteration": n,
+            "batched_fwd_rays_per_s": n / wf, "batched_fwd_s": wf,
+            "fit_iteration_rays_per_s": n / wg, "fit_iteration_s": wg,
+            "fit_iteration": "HIP graph: batched fwd + L2 loss grad + batched adjoint + Adam",
+            "plain_fwd_bwd_rays_per_s": n / wp, "plain_fwd_bwd_s": wp,
+            "plain": f"{n_instances} x (render + autograd backward wrt tilts)",
+            "batched_vs_plain_fwd_bwd": wp / wg, "loss_at_start": loss}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +188,8 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the C3 forward+backward line")
+                    help="skip the C3 forward+backward and C5 batched lines")
+    ap.add_argument("--c5-instances", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (for counter calibration)")
     args = ap.parse_args()
@@ -218,7 +268,8 @@ def main():
         _sync()
         out["copy_calibration_bytes"] = {"fetch": a.numel() * 4, "write": b.numel() * 4}
     if not args.no_extras and args.config == "C2":
-        out["extras"] = {"C3_fwd_bwd": c3_fwd_bwd(ia, configs, 3, 1)}
+        out["extras"] = {"C3_fwd_bwd": c3_fwd_bwd(ia, configs, 3, 1),
+                         "C5_batched": c5_batched(ia, configs, args.c5_instances)}
     if not args.no_cpu:
         cb, cimg, n = cpu_baseline(wl, tel)
         out["cpu_baseline"] = cb

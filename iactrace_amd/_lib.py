@@ -29,6 +29,9 @@ SOURCE_PARALLEL = 1
 MODE_IMAGE = 0
 MODE_RESPONSE = 1
 MODE_DEBUG = 2
+LAYOUT_AUTO = 0
+LAYOUT_SAMPLES = 1
+LAYOUT_SOURCES = 2
 SENSOR_SQUARE = 0
 SENSOR_HEX = 1
 APERTURE_DISK = 0
@@ -82,7 +85,8 @@ class IactScene(Structure):
 class IactLaunch(Structure):
     _fields_ = [("block_threads", c_int32), ("source_tile", c_int32),
                 ("blocks_per_cu", c_int32), ("deterministic", c_int32),
-                ("accumulate", c_int32)]
+                ("accumulate", c_int32), ("chunk_block", c_int32), ("block_order", c_int32),
+                ("layout", c_int32), ("sample_group", c_int32)]
 
 
 class IactGrads(Structure):

@@ -24,7 +24,7 @@ from .telescope import Telescope
 
 __all__ = ["TelescopeConfigBuilder", "hex_lattice", "hex_facet_vertices", "davies_cotton_dish",
            "parabolic_dish", "hex_camera", "cassegrain", "point_source_field", "Workload",
-           "workload"]
+           "workload", "BatchWorkload", "batch_workload", "parallel_grid"]
 
 
 class TelescopeConfigBuilder:
@@ -308,3 +308,51 @@ def workload(name, n_sources=None, n_samples=None):
         vals = np.ones(len(src), np.float32)
         return Workload(name, mirrors, [], sensor, n_samples or 2048, src, vals, "parallel")
     raise KeyError(name)
+
+
+@dataclass
+class BatchWorkload:
+    """C5: instances of the C3 dish, each with its own 0.5 mrad tilt draw
+    (numpy PCG64 seeds tilt_seed0 + i) and its own point-source field of
+    N_i ~ U{n_min..n_max} sources (sizes from seed 6, field of instance i
+    from seed source_seed0 + i).  The fit target of an instance is the
+    render of the untilted dish with that instance's sources."""
+    base: Workload
+    n_instances: int
+    sources: np.ndarray      # (total, 3) packed
+    values: np.ndarray       # (total,)
+    offsets: np.ndarray      # (n_instances + 1,) int64
+    tilt_seed0: int = 100
+
+    @property
+    def n_rays(self):
+        return self.base.n_facets * self.base.n_samples * int(self.offsets[-1])
+
+    def telescope(self, device="cuda", key=0):
+        """The untilted dish (sampled with ``key``)."""
+        return self.base.telescope(device, key)
+
+    def rotations(self, tel):
+        """(n_instances, n_facets, 3) stage-0 rotations with the tilt draws."""
+        base = tel.mirror_groups[0].rotations.detach().float()
+        out = base.unsqueeze(0).repeat(self.n_instances, 1, 1)
+        sig = np.rad2deg(0.5e-3)
+        for i in range(self.n_instances):
+            d = np.random.default_rng(self.tilt_seed0 + i).normal(0.0, sig, (base.shape[0], 2))
+            out[i, :, 0:2] += torch.as_tensor(d, dtype=torch.float32, device=base.device)
+        return out.contiguous()
+
+
+def batch_workload(n_instances=64, n_min=256, n_max=16384, n_samples=None, source_seed0=200):
+    """The C5 configuration (sizes may be reduced for tests)."""
+    base = workload("C3", n_samples=n_samples)
+    base.tilt_sigma_deg = 0.0
+    sizes = np.random.default_rng(6).integers(n_min, n_max + 1, n_instances)
+    src, vals = [], []
+    for i, n in enumerate(sizes):
+        a, b = point_source_field(int(n), 2.5, seed=source_seed0 + i)
+        src.append(a)
+        vals.append(b)
+    off = np.zeros(n_instances + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    return BatchWorkload(base, n_instances, np.concatenate(src, 0), np.concatenate(vals, 0), off)

@@ -233,21 +233,56 @@ void fill_trace_args(TraceArgs& A, const iact_scene& sc, const double* sources,
   A.ob = sc.obstructions;
   A.S = make_dev_sensor(sc.sensor);
   A.max_rounds = 16;
+  A.n_inst = 1;
 }
 
-// Resolve the launch shape of an image/debug trace.  Returns bytes of LDS.
+// Work order of the trace kernels: chunk-block size and the multipliers of
+// the item decode; `inst0` describes a plain render's single instance.
+int work_order(const iact_launch* L, long long nchunks, int grid, int wpb, TraceArgs& A) {
+  const int cb = L ? L->chunk_block : 0;
+  const int bo = L ? L->block_order : 0;
+  if (cb < -1) return set_error(IACT_ERR_INVALID, "chunk_block %d", cb);
+  if (bo < 0 || bo > 1) return set_error(IACT_ERR_INVALID, "block_order %d", bo);
+  if (nchunks >= (1ll << 31))
+    return set_error(IACT_ERR_UNSUPPORTED, "%lld sample chunks (max 2^31)", nchunks);
+  long long span = cb == 0 ? (long long)grid * wpb : (cb < 0 ? nchunks : cb);
+  if (span > nchunks) span = nchunks;
+  if (span < 1) span = 1;
+  const long long nc = nchunks > 0 ? nchunks : 1;
+  const long long n_cb = (nc + span - 1) / span;
+  A.span = (uint32_t)span;
+  A.nchunks = (uint32_t)nchunks;
+  A.d_span = make_fastdiv((uint32_t)span);
+  A.d_last = make_fastdiv((uint32_t)(nc - (n_cb - 1) * span));
+  A.d_cpf = make_fastdiv((uint32_t)(A.cpf > 0 ? A.cpf : 1));
+  A.inst0.nst = A.n_stiles;
+  A.inst0.ns = A.ns;
+  A.inst0.sbase = 0;
+  A.inst0.pb = make_fastdiv((uint32_t)(span * (A.n_stiles > 0 ? A.n_stiles : 1)));
+  A.xcd_order = bo == 0;
+  return IACT_OK;
+}
+
 int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, TraceArgs& A,
-                LaunchShape& out, const DevInfo& dev) {
+                LaunchShape& out, const DevInfo& dev, int source_type) {
   const int threads = (L && L->block_threads) ? L->block_threads : 256;
   if (threads % 64 || threads < 64 || threads > IACT_MAX_BLOCK)
     return set_error(IACT_ERR_INVALID, "block_threads %d must be a multiple of 64 in [64,%d]",
                      threads, IACT_MAX_BLOCK);
   const int wpb = threads / 64;
-  const int bpc = (L && L->blocks_per_cu) ? L->blocks_per_cu : 8;
-  if (bpc < 1 || bpc > 64) return set_error(IACT_ERR_INVALID, "blocks_per_cu %d", bpc);
+  int bpc = (L && L->blocks_per_cu) ? L->blocks_per_cu : 0;
+  if (bpc < 0 || bpc > 64) return set_error(IACT_ERR_INVALID, "blocks_per_cu %d", bpc);
   int ts = (L && L->source_tile) ? L->source_tile : 256;
   if (ts < 1) return set_error(IACT_ERR_INVALID, "source_tile %d", ts);
   if (ts > ns) ts = ns > 0 ? ns : 1;
+  // items of one instance (chunks x source tiles) are decoded in 32 bits
+  const long long nch = (long long)A.nf * A.cpf;
+  if (nch > 0 && ns > 0) {
+    const long long max_tiles = ((1ll << 31) - 1) / nch;
+    if (max_tiles < 1)
+      return set_error(IACT_ERR_UNSUPPORTED, "%lld sample chunks (max 2^31)", nch);
+    if ((ns + ts - 1) / ts > max_tiles) ts = (int)((ns + max_tiles - 1) / max_tiles);
+  }
   A.source_tile = ts;
   A.n_stiles = ns > 0 ? (ns + ts - 1) / ts : 0;
   A.det = (L && L->deterministic) ? 1 : 0;
@@ -279,22 +314,66 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
   out.lds = img + tab;
   const long long nchunks = (long long)A.nf * A.cpf;
   A.n_items = nchunks * A.n_stiles;
-  long long grid;
-  if (mode == IACT_MODE_RESPONSE)
-    grid = ns;
-  else
-    grid = (A.n_items + wpb - 1) / wpb;
+  // layout of image renders
+  const int lay = L ? L->layout : 0;
+  if (lay < 0 || lay > 2) return set_error(IACT_ERR_INVALID, "layout %d", lay);
+  A.layout = 1;
+  if (mode == IACT_MODE_IMAGE && lay == 2 && (A.det || sc.n_stages > 0))
+    return set_error(IACT_ERR_UNSUPPORTED,
+                     "the source-major layout covers primary-only, non-deterministic renders");
+  if (mode == IACT_MODE_IMAGE && !A.det && sc.n_stages == 0 && (lay == 2 || (lay == 0 && ns >= 64)))
+    A.layout = 2;
+  if (bpc == 0) bpc = default_blocks_per_cu(A.det);
   const long long cap = (long long)dev.n_cu * bpc;
+  long long grid;
+  if (A.layout == 2) {
+    const long long total = (long long)A.nf * A.M;
+    long long sg = (L && L->sample_group) ? L->sample_group : 64;
+    if (sg < 1) return set_error(IACT_ERR_INVALID, "sample_group %lld", sg);
+    const long long n_sc = ns > 0 ? (ns + 63) / 64 : 1;
+    // items are decoded in 32 bits
+    while (((total + sg - 1) / sg) * n_sc >= (1ll << 31)) sg *= 2;
+    A.sample_group = (int)sg;
+    A.n_sc = (uint32_t)n_sc;
+    A.d_sc = make_fastdiv((uint32_t)n_sc);
+    A.sm_items = ((total + sg - 1) / sg) * n_sc;
+    grid = (A.sm_items + wpb - 1) / wpb;
+  } else if (mode == IACT_MODE_RESPONSE) {
+    grid = ns;
+  } else {
+    grid = (A.n_items + wpb - 1) / wpb;
+  }
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
   out.grid = (int)grid;
-  return IACT_OK;
+  return work_order(L, nchunks, (int)grid, wpb, A);
+}
+
+// Default persistent grid: 64 blocks per CU for the trace kernels -- several
+// rounds of blocks, which the hardware hands to CUs as earlier ones finish,
+// balance the uneven cost of work items (measured faster than a grid of
+// resident blocks only); deterministic renders keep 8, their workspace
+// holding one partial image per block.
+int default_blocks_per_cu(int det) { return det ? 8 : 64; }
+
+int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShape& shp,
+           TraceArgs& A) {
+  const int bpc = (L && L->blocks_per_cu) ? L->blocks_per_cu : default_blocks_per_cu(0);
+  const int wpb = shp.threads / 64;
+  long long grid = (items + wpb - 1) / wpb;
+  const long long cap = (long long)dev.n_cu * bpc;
+  if (grid > cap) grid = cap;
+  if (grid < 1) grid = 1;
+  shp.grid = (int)grid;
+  return work_order(L, (long long)A.nf * A.cpf, shp.grid, wpb, A);
 }
 
 size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
                            const TraceArgs& A) {
   if (mode == IACT_MODE_RESPONSE)
     return (size_t)7 * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
+  if (mode == IACT_MODE_IMAGE && A.layout == 2)
+    return (size_t)IACT_REC_WORDS * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
   if (mode == IACT_MODE_IMAGE && A.det) return (size_t)shp.grid * sc.sensor.n_pixels * sizeof(float);
   (void)ns;
   return 0;
@@ -414,7 +493,7 @@ size_t iact_workspace_size(const iact_scene* scene, int32_t n_sources, int32_t m
   TraceArgs A;
   fill_trace_args(A, *scene, nullptr, nullptr, n_sources);
   LaunchShape shp;
-  if (shape_trace(*scene, n_sources, mode, launch, A, shp, dev)) return 0;
+  if (shape_trace(*scene, n_sources, mode, launch, A, shp, dev, -1)) return 0;
   return workspace_bytes_for(*scene, n_sources, mode, shp, A);
 }
 
@@ -446,7 +525,7 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
   TraceArgs A;
   fill_trace_args(A, sc, sources, values, n_sources);
   LaunchShape shp;
-  rc = shape_trace(sc, n_sources, mode, launch, A, shp, dev);
+  rc = shape_trace(sc, n_sources, mode, launch, A, shp, dev, source_type);
   if (rc) return rc;
   const size_t need = workspace_bytes_for(sc, n_sources, mode, shp, A);
   if (need > 0 && (!workspace || workspace_bytes < need))
@@ -463,6 +542,13 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
       if (e != hipSuccess) return hip_error(e, "clear image");
     }
     if (empty) return IACT_OK;
+    if (A.layout == 2) {
+      A.recs = (const float*)workspace;
+      e = launch_records(A, (float*)workspace, st);
+      if (e != hipSuccess) return hip_error(e, "k_sample_records");
+      e = launch_trace_sm(A, shp, source_type, st);
+      return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_sm");
+    }
     A.partials = (float*)workspace;
     e = launch_trace(A, shp, source_type, mode, st);
     if (e != hipSuccess) return hip_error(e, "k_trace");

@@ -11,8 +11,14 @@ struct DevInfo {
   char arch[64] = {0};
 };
 int device_info(DevInfo& out);
+int work_order(const iact_launch* L, long long nchunks, int grid, int wpb, TraceArgs& A);
 int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, TraceArgs& A,
-                LaunchShape& out, const DevInfo& dev);
+                LaunchShape& out, const DevInfo& dev, int source_type);
+int default_blocks_per_cu(int det);
+// Persistent grid for `items` wave-sized work items at the shape's block
+// size; updates the work order that depends on the grid.
+int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShape& shp,
+           TraceArgs& A);
 int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* where);
 int validate_scene(const iact_scene* sc);

@@ -19,18 +19,6 @@
 
 namespace iact {
 
-struct AdjArgs {
-  TraceArgs T;
-  const float* grad_image;
-  double* g_rot;
-  double* g_pos;
-  double* g_src;
-  double* g_val;
-  double* g_asph;
-  const int32_t* surface_index;
-  const float* offsets;
-};
-
 // dR/d(angle) of R = Rz(c) Ry(b) Rx(a) for each of the three angles (radians).
 IACT_DEV void euler_derivs(float tip, float tilt, float rot, m3& Da, m3& Db, m3& Dc) {
   const float k = 0.017453292519943295f;
@@ -179,7 +167,11 @@ IACT_DEV float splat_adjoint(const DevSensor& S, const int32_t* table, const flo
   }
 }
 
-template <int SRC, int KIND, int SPLAT>
+// BATCH: many instances (TraceArgs batch fields); the LDS holds the image
+// gradient of the instance the block is working on and is reloaded when the
+// block moves on to the next instance; only rotation gradients, per
+// instance, are produced.
+template <int SRC, int KIND, int SPLAT, bool BATCH>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   const TraceArgs& A = B.T;
   extern __shared__ float smem[];
@@ -192,19 +184,31 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
   const int lane = lane_id();
-  const long long nchunks = (long long)A.nf * A.cpf;
-  const bool want_src = B.g_src != nullptr, want_val = B.g_val != nullptr;
-  const bool want_asph = B.g_asph != nullptr && B.surface_index != nullptr && B.offsets != nullptr;
+  const bool want_src = !BATCH && B.g_src != nullptr, want_val = !BATCH && B.g_val != nullptr;
+  const bool want_asph = !BATCH && B.g_asph != nullptr && B.surface_index != nullptr &&
+                         B.offsets != nullptr;
+  const long long n_items = total_items(A);
+  const long long n_bitems = (n_items + wpb - 1) / wpb;
+  int cur = 0;  // instance whose image gradient the LDS holds
 
-  for (long long item = (long long)blockIdx.x * wpb + wave; item < A.n_items;
-       item += (long long)gridDim.x * wpb) {
-    const long long chunk = item % nchunks;
-    const int stile = (int)(item / nchunks);
-    const int f = (int)(chunk / A.cpf);
-    const int m = (int)(chunk % A.cpf) * 64 + lane;
+  for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += gridDim.x) {
+    const int inst = item_instance(A, bi * wpb);  // block-uniform
+    if (BATCH && inst != cur) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < P; i += blockDim.x)
+        G[i] = B.grad_image[(size_t)inst * P + i];
+      __syncthreads();
+    }
+    cur = inst;
+    const long long item = bi * wpb + wave;
+    WorkItem w;
+    if (item >= n_items || !decode_item(A, item, inst, lane, w)) continue;
+    const int f = w.f, m = w.m;
     const bool active = m < A.M;
     const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
-    const m3 R = facet_rotation(A.rotations, f);
+    const float* rot = A.rotations + w.inst * A.rot_stride;
+    double* const g_rot = B.g_rot ? B.g_rot + (size_t)w.inst * 3 * A.nf : nullptr;
+    const m3 R = facet_rotation(rot, f);
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
     const f3 pl = mk3(A.points[3 * gi], A.points[3 * gi + 1], A.points[3 * gi + 2]);
     const f3 nl = mk3(A.normals[3 * gi], A.normals[3 * gi + 1], A.normals[3 * gi + 2]);
@@ -216,10 +220,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     const Sample smp = make_sample(A.S, pw, nw, wgt);
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
-    const int s0 = stile * A.source_tile;
-    const int s1 = min(s0 + A.source_tile, A.ns);
-    const cptr<double> src = as_const(A.sources);
-    const cptr<float> vals = as_const(A.values);
+    const int s0 = w.s0, s1 = w.s1;
+    const cptr<double> src = as_const(A.sources + 3 * w.sbase);
+    const cptr<float> vals = as_const(A.values + w.sbase);
     for (int s = s0; s < s1; ++s) {
       const double sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
       const float value = vals[s];
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     const f3 pert = mul(R, nls);
     const float ipn = frsq(dot(pert, pert));
     const f3 gpert = ipn * fma3(-dot(nw, An), nw, An);
-    if (B.g_rot) {
+    if (g_rot) {
       float dR[9];
       dR[0] = fmaf(Ap.x, pl.x, gpert.x * nls.x);
       dR[1] = fmaf(Ap.x, pl.y, gpert.x * nls.y);
@@ -314,19 +317,18 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       dR[7] = fmaf(Ap.z, pl.y, gpert.z * nls.y);
       dR[8] = fmaf(Ap.z, pl.z, gpert.z * nls.z);
       m3 Da, Db, Dc;
-      euler_derivs(A.rotations[3 * f], A.rotations[3 * f + 1], A.rotations[3 * f + 2], Da, Db,
-                   Dc);
+      euler_derivs(rot[3 * f], rot[3 * f + 1], rot[3 * f + 2], Da, Db, Dc);
       const float k = 0.017453292519943295f;
       const float ga = wave_sum(frob(Da, dR) * k);
       const float gb = wave_sum(frob(Db, dR) * k);
       const float gc = wave_sum(frob(Dc, dR) * k);
       if (lane == 0) {
-        unsafeAtomicAdd(B.g_rot + 3 * f, (double)ga);
-        unsafeAtomicAdd(B.g_rot + 3 * f + 1, (double)gb);
-        unsafeAtomicAdd(B.g_rot + 3 * f + 2, (double)gc);
+        unsafeAtomicAdd(g_rot + 3 * f, (double)ga);
+        unsafeAtomicAdd(g_rot + 3 * f + 1, (double)gb);
+        unsafeAtomicAdd(g_rot + 3 * f + 2, (double)gc);
       }
     }
-    if (B.g_pos) {
+    if (!BATCH && B.g_pos) {
       const float a = wave_sum(Ap.x), b = wave_sum(Ap.y), cz = wave_sum(Ap.z);
       if (lane == 0) {
         unsafeAtomicAdd(B.g_pos + 3 * f, (double)a);
@@ -369,27 +371,33 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   }
 }
 
+using AdjKernel = void (*)(AdjArgs);
+
 template <int SRC, int KIND, int SPLAT>
-static hipError_t launch_adj_t(const AdjArgs& B, const LaunchShape& L, hipStream_t st) {
-  hipLaunchKernelGGL((k_trace_adj<SRC, KIND, SPLAT>), dim3(L.grid), dim3(L.threads), L.lds, st,
-                     B);
-  return hipGetLastError();
+static AdjKernel pick_adj_t(const AdjArgs& B) {
+  return B.T.item_prefix ? k_trace_adj<SRC, KIND, SPLAT, true> : k_trace_adj<SRC, KIND, SPLAT, false>;
 }
 
 template <int SRC>
-static hipError_t dispatch_adj(const AdjArgs& B, const LaunchShape& L, hipStream_t st) {
+static AdjKernel pick_adj_src(const AdjArgs& B) {
   const DevSensor& S = B.T.S;
   if (S.kind == IACT_SENSOR_SQUARE)
-    return S.splat ? launch_adj_t<SRC, IACT_SENSOR_SQUARE, 1>(B, L, st)
-                   : launch_adj_t<SRC, IACT_SENSOR_SQUARE, 0>(B, L, st);
-  return S.splat ? launch_adj_t<SRC, IACT_SENSOR_HEX, 1>(B, L, st)
-                 : launch_adj_t<SRC, IACT_SENSOR_HEX, 0>(B, L, st);
+    return S.splat ? pick_adj_t<SRC, IACT_SENSOR_SQUARE, 1>(B)
+                   : pick_adj_t<SRC, IACT_SENSOR_SQUARE, 0>(B);
+  return S.splat ? pick_adj_t<SRC, IACT_SENSOR_HEX, 1>(B) : pick_adj_t<SRC, IACT_SENSOR_HEX, 0>(B);
+}
+
+const void* pick_adjoint_kernel(const AdjArgs& B, int source_type) {
+  return (const void*)(source_type == IACT_SOURCE_POINT ? pick_adj_src<IACT_SOURCE_POINT>(B)
+                                                        : pick_adj_src<IACT_SOURCE_PARALLEL>(B));
 }
 
 hipError_t launch_adjoint(const AdjArgs& B, const LaunchShape& L, int source_type,
                           hipStream_t st) {
-  return source_type == IACT_SOURCE_POINT ? dispatch_adj<IACT_SOURCE_POINT>(B, L, st)
-                                          : dispatch_adj<IACT_SOURCE_PARALLEL>(B, L, st);
+  AdjArgs b = B;
+  void* args[] = {&b};
+  return hipLaunchKernel(pick_adjoint_kernel(B, source_type), dim3(L.grid), dim3(L.threads),
+                         args, L.lds, st);
 }
 
 }  // namespace iact
@@ -446,7 +454,7 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   memset(&B, 0, sizeof(B));
   fill_trace_args(B.T, sc, sources, values, n_sources);
   LaunchShape shp;
-  rc = shape_trace(sc, n_sources, IACT_MODE_DEBUG, launch, B.T, shp, dev);
+  rc = shape_trace(sc, n_sources, IACT_MODE_DEBUG, launch, B.T, shp, dev, -1);
   if (rc) return rc;
   const size_t lds = (size_t)sc.sensor.n_pixels * sizeof(float) + shp.lds;
   if (lds > 65536)
@@ -460,6 +468,8 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   B.g_asph = grads->aspheric;
   B.surface_index = sc.facets.surface_index;
   B.offsets = sc.facets.offsets;
+  rc = regrid(B.T.n_items, launch, dev, shp, B.T);
+  if (rc) return rc;
   e = launch_adjoint(B, shp, source_type, st);
   return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_adj");
 }

@@ -18,10 +18,35 @@ hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t st);
 hipError_t launch_sample(const iact_sample_desc& D, float* p, float* n, float* d, float* w,
                          hipStream_t st);
 hipError_t launch_world(const iact_facets& F, const DevSensor& S, float* out, hipStream_t st);
+// The trace kernels, by role.
+enum TraceRole { TRACE_IMAGE = 0, TRACE_DEBUG = 1, TRACE_RESPONSE = 2, TRACE_SOURCE_MAJOR = 3 };
+using TraceKernel = void (*)(TraceArgs);
+TraceKernel pick_trace_kernel(const TraceArgs& A, int source_type, int which);
+hipError_t launch_trace_kernel(TraceKernel k, const TraceArgs& A, const LaunchShape& L,
+                               hipStream_t st);
 hipError_t launch_trace(const TraceArgs& A, const LaunchShape& L, int source_type, int mode,
                         hipStream_t st);
 hipError_t launch_accumulate(const DevSensor& S, const float* x, const float* y, const float* v,
                              long long n, float* out, int32_t* pixel, hipStream_t st);
+hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_type,
+                           hipStream_t st);
+hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st);
 hipError_t launch_reduce(const float* partials, int nblocks, int P, float* out, hipStream_t st);
+
+// Arguments of the adjoint kernel (iact_adjoint.hip).
+struct AdjArgs {
+  TraceArgs T;
+  const float* grad_image;  // (P), or (n_inst, P) for batched renders
+  double* g_rot;            // (nf, 3), or (n_inst, nf, 3) for batched renders
+  double* g_pos;
+  double* g_src;
+  double* g_val;
+  double* g_asph;
+  const int32_t* surface_index;
+  const float* offsets;
+};
+hipError_t launch_adjoint(const AdjArgs& B, const LaunchShape& L, int source_type,
+                          hipStream_t st);
+const void* pick_adjoint_kernel(const AdjArgs& B, int source_type);
 
 }  // namespace iact

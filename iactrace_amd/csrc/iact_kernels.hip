@@ -202,27 +202,41 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   __syncthreads();
   const int32_t* table = tab_lds;  // hex tables are always staged in LDS
   float* img = img_lds + (A.det ? wave * P : 0);
-  const long long nchunks = (long long)A.nf * A.cpf;
 
   int run_pix = -1;
   float run_v = 0.f;
-  for (long long item = (long long)blockIdx.x * wpb + wave; item < A.n_items;
-       item += (long long)gridDim.x * wpb) {
-    const long long chunk = item % nchunks;
-    const int stile = (int)(item / nchunks);
-    const int f = (int)(chunk / A.cpf);
-    const int m = (int)(chunk % A.cpf) * 64 + lane;
+  const long long n_items = total_items(A);
+  const long long n_bitems = (n_items + wpb - 1) / wpb;
+  int cur = 0;  // instance whose image the LDS holds
+  // the waves of a block take consecutive items, which belong to one
+  // instance; everything down to `continue` is block-uniform
+  for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += gridDim.x) {
+    const int inst = item_instance(A, bi * wpb);
+    if (MODE == IACT_MODE_IMAGE && A.lds_image && inst != cur) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const float v = img_lds[i];
+        if (v != 0.f) unsafeAtomicAdd(A.out + (size_t)cur * P + i, v);
+        img_lds[i] = 0.f;
+      }
+      __syncthreads();
+    }
+    cur = inst;
+    const long long item = bi * wpb + wave;
+    WorkItem w;
+    if (item >= n_items || !decode_item(A, item, inst, lane, w)) continue;
+    const int f = w.f, m = w.m;
     const bool active = m < A.M;
     const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
-    const m3 R = facet_rotation(A.rotations, f);
+    const m3 R = facet_rotation(A.rotations + w.inst * A.rot_stride, f);
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
     const Sample smp = make_sample(A.S, pw, nw, A.weights[gi]);
-    const int s0 = stile * A.source_tile;
-    const int s1 = min(s0 + A.source_tile, A.ns);
-    const cptr<double> src = as_const(A.sources);
-    const cptr<float> vals = as_const(A.values);
+    const int s0 = w.s0, s1 = w.s1;
+    float* const gout = A.out + (size_t)w.inst * P;
+    const cptr<double> src = as_const(A.sources + 3 * w.sbase);
+    const cptr<float> vals = as_const(A.values + w.sbase);
     double sx = src[3 * s0], sy = src[3 * s0 + 1], sz = src[3 * s0 + 2];
     float sv = vals[s0];
     for (int s = s0; s < s1; ++s) {
@@ -259,14 +273,14 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
         } else if (A.lds_image) {
           wave_accumulate<false>(img, pix, v, A.max_rounds);
         } else {
-          wave_accumulate<true>(A.out, pix, v, A.max_rounds);
+          wave_accumulate<true>(gout, pix, v, A.max_rounds);
         }
       } else {
-        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
+        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float wt) {
           if (A.lds_image)
-            wave_accumulate<false>(img, pix, v * w, A.max_rounds);
+            wave_accumulate<false>(img, pix, v * wt, A.max_rounds);
           else
-            wave_accumulate<true>(A.out, pix, v * w, A.max_rounds);
+            wave_accumulate<true>(gout, pix, v * wt, A.max_rounds);
         });
       }
     }
@@ -284,8 +298,115 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     } else {
       for (int i = threadIdx.x; i < P; i += blockDim.x) {
         const float v = img_lds[i];
-        if (v != 0.f) unsafeAtomicAdd(A.out + i, v);
+        if (v != 0.f) unsafeAtomicAdd(A.out + (size_t)cur * P + i, v);
       }
+    }
+  }
+}
+
+// World-space records of every (facet, sample) for the source-major trace.
+__global__ void k_sample_records(TraceArgs A, float* recs) {
+  const long long total = (long long)A.nf * A.M;
+  for (long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x; gi < total;
+       gi += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(gi / A.M);
+    const m3 R = facet_rotation(A.rotations, f);
+    const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
+    f3 pw, nw;
+    world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], (size_t)gi, pw, nw);
+    store_record(recs + IACT_REC_WORDS * gi, make_sample(A.S, pw, nw, A.weights[gi]));
+  }
+}
+
+// Image trace, source-major layout: lane = one source, a work item is a group
+// of `sample_group` consecutive samples (facet-major) traced against a chunk
+// of 64 sources; the sample's world-space record is wave-uniform (scalar
+// loads, prefetched one sample ahead).  Rays of a wave belong to different
+// sources and land on different pixels; with hard binning each lane sums its
+// consecutive rays that hit the same pixel (the samples of one facet put a
+// source's rays on one or two pixels) and adds the run to the LDS image when
+// the pixel changes; splat taps are added one by one.  The per-ray arithmetic
+// is trace_ray's, as in the sample-major kernel.
+template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS>
+__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
+  extern __shared__ float smem[];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+  const int lane = lane_id();
+  const int P = A.S.n_pixels;
+  float* img = smem;
+  int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? P : 0));
+  if (A.lds_image)
+    for (int i = threadIdx.x; i < P; i += blockDim.x) img[i] = 0.f;
+  if (KIND == IACT_SENSOR_HEX && A.lds_table) {
+    const int n = A.S.tq * A.S.tr;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
+  }
+  __syncthreads();
+  const int32_t* table = tab_lds;
+  float* const acc = A.lds_image ? img : A.out;
+  const long long total = (long long)A.nf * A.M;
+  for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
+       item += (long long)gridDim.x * wpb) {
+    // source chunk fastest: consecutive waves share the sample group
+    const uint32_t g = fdiv((uint32_t)item, A.d_sc);
+    const int c = (int)((uint32_t)item - g * A.n_sc);
+    const int s = c * 64 + lane;
+    const bool active = s < A.ns;
+    const int sl = active ? s : A.ns - 1;
+    const double sx = A.sources[3 * sl], sy = A.sources[3 * sl + 1], sz = A.sources[3 * sl + 2];
+    const float value = A.values[sl];
+    const long long j0 = (long long)g * A.sample_group;
+    const long long j1 = min(j0 + A.sample_group, total);
+    const float4* recs = (const float4*)A.recs;
+    RecRaw cur_r = fetch_record(recs + 4 * j0);
+    int run_pix = -1;
+    float run_v = 0.f;
+    for (long long j = j0; j < j1; ++j) {
+      const long long jn = j + 1 < j1 ? j + 1 : j;
+      const RecRaw nxt_r = fetch_record(recs + 4 * jn);
+      const Sample smp = unpack_record(cur_r);
+      float x, y, v;
+      const bool ok = trace_ray<SRC, STAGES, AXIS>(A, smp, active, sx, sy, sz, value, x, y, v);
+      if (!SPLAT) {
+        // run of equal pixels per lane (consecutive samples of a facet put a
+        // source's rays on the same one or two pixels), added when it ends
+        const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        if (pix != run_pix) {
+          if (run_pix >= 0 && run_v != 0.f) {
+            if (A.lds_image)
+              atomicAdd(acc + run_pix, run_v);
+            else
+              unsafeAtomicAdd(acc + run_pix, run_v);
+          }
+          run_pix = pix;
+          run_v = 0.f;
+        }
+        run_v += pix >= 0 ? v : 0.f;
+      } else {
+        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
+          const float vw = v * w;
+          if (pix >= 0 && vw != 0.f) {
+            if (A.lds_image)
+              atomicAdd(acc + pix, vw);
+            else
+              unsafeAtomicAdd(acc + pix, vw);
+          }
+        });
+      }
+      cur_r = nxt_r;
+    }
+    if (!SPLAT && run_pix >= 0 && run_v != 0.f) {
+      if (A.lds_image)
+        atomicAdd(acc + run_pix, run_v);
+      else
+        unsafeAtomicAdd(acc + run_pix, run_v);
+    }
+  }
+  if (A.lds_image) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+      const float v = img[i];
+      if (v != 0.f) unsafeAtomicAdd(A.out + i, v);
     }
   }
 }
@@ -381,57 +502,82 @@ hipError_t launch_world(const iact_facets& F, const DevSensor& S, float* out, hi
   return hipGetLastError();
 }
 
+// ---- kernel selection.  Every trace launch and every occupancy query goes
+// through pick_trace_kernel, so the grid is sized for the kernel that runs.
 template <int SRC, int KIND, int SPLAT, int MODE>
-static hipError_t launch_trace_t(const TraceArgs& A, const LaunchShape& L, hipStream_t st) {
+static TraceKernel pick_trace_t(const TraceArgs& A) {
   // image mode is specialised for primary-only scenes and identity sensor
   // frames; both specialisations compute bit-identical results to the
   // generic kernel, they only skip instructions
-  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0) {
-    if (A.S.axis)
-      hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, false, true>), dim3(L.grid),
-                         dim3(L.threads), L.lds, st, A);
-    else
-      hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, false, false>), dim3(L.grid),
-                         dim3(L.threads), L.lds, st, A);
-  } else {
-    hipLaunchKernelGGL((k_trace<SRC, KIND, SPLAT, MODE, true, false>), dim3(L.grid),
-                       dim3(L.threads), L.lds, st, A);
-  }
-  return hipGetLastError();
+  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0)
+    return A.S.axis ? k_trace<SRC, KIND, SPLAT, MODE, false, true>
+                    : k_trace<SRC, KIND, SPLAT, MODE, false, false>;
+  return k_trace<SRC, KIND, SPLAT, MODE, true, false>;
 }
 
+// source-major kernels exist for primary-only scenes (the host selects that
+// layout only for those)
 template <int SRC, int KIND, int SPLAT>
-static hipError_t launch_resp_t(const TraceArgs& A, const LaunchShape& L, hipStream_t st) {
-  hipLaunchKernelGGL((k_trace_response<SRC, KIND, SPLAT>), dim3(L.grid), dim3(L.threads), L.lds,
-                     st, A);
-  return hipGetLastError();
+static TraceKernel pick_sm_t(const TraceArgs& A) {
+  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true>
+                  : k_trace_sm<SRC, KIND, SPLAT, false, false>;
 }
 
 template <int SRC>
-static hipError_t dispatch_trace(const TraceArgs& A, const LaunchShape& L, int mode,
-                                 hipStream_t st) {
+static TraceKernel pick_src(const TraceArgs& A, int which) {
   const bool sq = A.S.kind == IACT_SENSOR_SQUARE;
   const bool sp = A.S.splat != 0;
-  if (mode == IACT_MODE_RESPONSE) {
-    if (sq) return sp ? launch_resp_t<SRC, IACT_SENSOR_SQUARE, 1>(A, L, st)
-                      : launch_resp_t<SRC, IACT_SENSOR_SQUARE, 0>(A, L, st);
-    return sp ? launch_resp_t<SRC, IACT_SENSOR_HEX, 1>(A, L, st)
-              : launch_resp_t<SRC, IACT_SENSOR_HEX, 0>(A, L, st);
+  switch (which) {
+    case TRACE_RESPONSE:
+      if (sq) return sp ? k_trace_response<SRC, IACT_SENSOR_SQUARE, 1>
+                        : k_trace_response<SRC, IACT_SENSOR_SQUARE, 0>;
+      return sp ? k_trace_response<SRC, IACT_SENSOR_HEX, 1>
+                : k_trace_response<SRC, IACT_SENSOR_HEX, 0>;
+    case TRACE_DEBUG:
+      return sq ? pick_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_DEBUG>(A)
+                : pick_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_DEBUG>(A);
+    case TRACE_SOURCE_MAJOR:
+      if (sq) return sp ? pick_sm_t<SRC, IACT_SENSOR_SQUARE, 1>(A)
+                        : pick_sm_t<SRC, IACT_SENSOR_SQUARE, 0>(A);
+      return sp ? pick_sm_t<SRC, IACT_SENSOR_HEX, 1>(A) : pick_sm_t<SRC, IACT_SENSOR_HEX, 0>(A);
+    default:
+      if (sq) return sp ? pick_trace_t<SRC, IACT_SENSOR_SQUARE, 1, IACT_MODE_IMAGE>(A)
+                        : pick_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_IMAGE>(A);
+      return sp ? pick_trace_t<SRC, IACT_SENSOR_HEX, 1, IACT_MODE_IMAGE>(A)
+                : pick_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_IMAGE>(A);
   }
-  if (mode == IACT_MODE_DEBUG) {
-    if (sq) return launch_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_DEBUG>(A, L, st);
-    return launch_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_DEBUG>(A, L, st);
-  }
-  if (sq) return sp ? launch_trace_t<SRC, IACT_SENSOR_SQUARE, 1, IACT_MODE_IMAGE>(A, L, st)
-                    : launch_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_IMAGE>(A, L, st);
-  return sp ? launch_trace_t<SRC, IACT_SENSOR_HEX, 1, IACT_MODE_IMAGE>(A, L, st)
-            : launch_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_IMAGE>(A, L, st);
+}
+
+TraceKernel pick_trace_kernel(const TraceArgs& A, int source_type, int which) {
+  return source_type == IACT_SOURCE_POINT ? pick_src<IACT_SOURCE_POINT>(A, which)
+                                          : pick_src<IACT_SOURCE_PARALLEL>(A, which);
+}
+
+hipError_t launch_trace_kernel(TraceKernel k, const TraceArgs& A, const LaunchShape& L,
+                               hipStream_t st) {
+  TraceArgs a = A;
+  void* args[] = {&a};
+  return hipLaunchKernel((const void*)k, dim3(L.grid), dim3(L.threads), args, L.lds, st);
 }
 
 hipError_t launch_trace(const TraceArgs& A, const LaunchShape& L, int source_type, int mode,
                         hipStream_t st) {
-  return source_type == IACT_SOURCE_POINT ? dispatch_trace<IACT_SOURCE_POINT>(A, L, mode, st)
-                                          : dispatch_trace<IACT_SOURCE_PARALLEL>(A, L, mode, st);
+  const int which = mode == IACT_MODE_RESPONSE ? TRACE_RESPONSE
+                    : mode == IACT_MODE_DEBUG  ? TRACE_DEBUG
+                                               : TRACE_IMAGE;
+  return launch_trace_kernel(pick_trace_kernel(A, source_type, which), A, L, st);
+}
+
+hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_type,
+                           hipStream_t st) {
+  return launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR), A, L, st);
+}
+
+hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st) {
+  const long long total = (long long)A.nf * A.M;
+  hipLaunchKernelGGL(k_sample_records, dim3(grid_for(total, 256, 16384)), dim3(256), 0, st, A,
+                     recs);
+  return hipGetLastError();
 }
 
 hipError_t launch_reduce(const float* partials, int nblocks, int P, float* out, hipStream_t st) {

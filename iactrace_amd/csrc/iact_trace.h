@@ -12,6 +12,41 @@
 
 namespace iact {
 
+// Unsigned 32-bit division by an invariant divisor d >= 1 (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI
+// 1994, fig. 4.1): q = (t + ((n - t) >> s1)) >> s2 with t = mulhi(m, n),
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, s1 = min(l, 1),
+// s2 = max(l - 1, 0).  Exact for every 32-bit n.
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv D;
+  D.d = d;
+  uint32_t l = 0;
+  while (l < 32 && (1ull << l) < d) ++l;
+  const unsigned __int128 num = ((unsigned __int128)1 << 32) * ((1ull << l) - d);
+  D.m = (uint32_t)(num / d + 1);
+  D.s1 = l > 0 ? 1u : 0u;
+  D.s2 = l > 0 ? l - 1 : 0u;
+  return D;
+}
+
+IACT_DEV uint32_t fdiv(uint32_t n, const FastDiv& D) {
+  const uint32_t t = __umulhi(D.m, n);
+  return (t + ((n - t) >> D.s1)) >> D.s2;
+}
+
+// Sizes of one instance of a render (items = nchunks * nst, padded).
+struct InstDesc {
+  FastDiv pb;          // divisor span * nst: items per full chunk block
+  int32_t nst, ns;     // source tiles, sources
+  long long sbase;     // first source in the packed source arrays
+};
+static_assert(sizeof(InstDesc) == 32 && offsetof(InstDesc, sbase) == 24,
+              "decode_item reads InstDesc as 8 words");
+
 struct TraceArgs {
   // stage-0 facets (local frame samples + pose)
   int nf, M, cpf;  // cpf = chunks of 64 samples per facet
@@ -47,7 +82,125 @@ struct TraceArgs {
   int det;             // deterministic (wave-private images, ordered sums)
   int max_rounds;
   int acc;             // 0 wave-merged, 1 per-lane run-length, 2 direct atomics
+  // work order (see decode_item)
+  uint32_t span;       // chunks per block of the order
+  uint32_t nchunks;    // nf * cpf
+  FastDiv d_span, d_last, d_cpf;  // divide by span, by the last block's size, by cpf
+  InstDesc inst0;      // the single instance of a plain render
+  // batched renders: n_inst telescope instances share the facet samples and
+  // carry their own stage-0 rotations, source lists and images
+  int n_inst;
+  const long long* item_prefix;  // (n_inst + 1) work-item prefix; NULL = one instance
+  const InstDesc* inst_desc;     // (n_inst) per-instance sizes
+  long long rot_stride;          // floats between two instances' rotations
+  int xcd_order;                 // 1: XCD-major block numbering
+  // source-major layout: lanes are sources, samples come from `recs`
+  int layout;                    // 1 sample-major, 2 source-major
+  const float* recs;             // (nf*M, IACT_REC_WORDS) world-space sample records
+  int sample_group;              // samples per work item
+  uint32_t n_sc;                 // source chunks of 64
+  FastDiv d_sc;                  // divide by n_sc
+  long long sm_items;            // sample groups * source chunks
 };
+
+// World-space sample record of the source-major trace (16 words, one scalar
+// load): p as 3 doubles, n, -1/weight, the sensor-plane constants of p, and p
+// in fp32 -- exactly the Sample that make_sample builds.
+#define IACT_REC_WORDS 16
+
+// Position of a block in an XCD-major order.  Workgroups are dispatched to
+// the 8 XCDs round-robin (block b runs on XCD b % 8), so consecutive logical
+// blocks share one XCD's L2.  A permutation of [0, gridDim.x).
+IACT_DEV long long xcd_block(int on) {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (!on) return b;
+  const int x = b & 7, per = G >> 3, rem = G & 7;
+  return (long long)x * per + min(x, rem) + (b >> 3);
+}
+
+// One work item: 64 consecutive samples of facet f (lane -> sample m) traced
+// against sources [s0, s1) of instance inst.
+//
+// Order within an instance: chunks come in blocks of `span` (by default the
+// waves of one grid sweep); inside a block the source tile is the slower
+// index and the chunk the faster.  One sweep of the persistent grid therefore
+// traces the chunks of a block against one source tile -- all waves read the
+// same sources -- and the next sweeps revisit the same chunks, which each XCD
+// keeps in its L2 (span / 8 chunks of 2.5 KB).  Instances are padded to a
+// multiple of the block's wave count so that the waves of a block always work
+// on the same instance.  Every division of the decode is by a divisor fixed
+// per launch or per instance, done with a precomputed multiplier: integer
+// division has no scalar hardware and costs a long vector sequence.
+struct WorkItem {
+  int inst, f, m, s0, s1;
+  long long sbase;       // first source of the instance in the source arrays
+};
+
+IACT_DEV long long total_items(const TraceArgs& A) {
+  return A.item_prefix ? as_const(A.item_prefix)[A.n_inst] : A.n_items;
+}
+
+// Instance of an item (0 for single renders).
+IACT_DEV int item_instance(const TraceArgs& A, long long item) {
+  if (!A.item_prefix) return 0;
+  const cptr<long long> ip = as_const(A.item_prefix);
+  int lo = 0, hi = A.n_inst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ip[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// false for the padding items at the end of an instance.
+IACT_DEV bool decode_item(const TraceArgs& A, long long item, int inst, int lane, WorkItem& w) {
+  long long local64 = item;
+  InstDesc I;
+  if (A.item_prefix) {
+    local64 = item - as_const(A.item_prefix)[inst];
+    // scalar loads of the 8 words of the instance's descriptor
+    const cptr<uint32_t> u = (cptr<uint32_t>)as_const(A.inst_desc + inst);
+    I.pb.d = u[0];
+    I.pb.m = u[1];
+    I.pb.s1 = u[2];
+    I.pb.s2 = u[3];
+    I.nst = (int32_t)u[4];
+    I.ns = (int32_t)u[5];
+    I.sbase = (long long)((uint64_t)u[6] | ((uint64_t)u[7] << 32));
+  } else {
+    I.pb.d = A.inst0.pb.d;
+    I.pb.m = A.inst0.pb.m;
+    I.pb.s1 = A.inst0.pb.s1;
+    I.pb.s2 = A.inst0.pb.s2;
+    I.nst = A.inst0.nst;
+    I.ns = A.inst0.ns;
+    I.sbase = A.inst0.sbase;
+  }
+  const uint32_t local = (uint32_t)local64;
+  if (local64 >= (long long)A.nchunks * I.nst) return false;
+  const uint32_t cb = fdiv(local, I.pb);
+  const uint32_t rem = local - cb * I.pb.d;
+  const uint32_t first = cb * A.span;
+  const bool full = A.nchunks - first >= A.span;
+  // the divisor picked field by field (a selected struct would go through
+  // the stack)
+  FastDiv Db;
+  Db.d = full ? A.d_span.d : A.d_last.d;
+  Db.m = full ? A.d_span.m : A.d_last.m;
+  Db.s1 = full ? A.d_span.s1 : A.d_last.s1;
+  Db.s2 = full ? A.d_span.s2 : A.d_last.s2;
+  const uint32_t stile = fdiv(rem, Db);
+  const uint32_t cbs = Db.d;
+  const uint32_t chunk = first + (rem - stile * cbs);
+  const uint32_t f = fdiv(chunk, A.d_cpf);
+  w.inst = inst;
+  w.sbase = I.sbase;
+  w.f = (int)f;
+  w.m = (int)(chunk - f * A.d_cpf.d) * 64 + lane;
+  w.s0 = (int)stile * A.source_tile;
+  w.s1 = min(w.s0 + A.source_tile, I.ns);
+  return true;
+}
 
 // One sample in world space, ready to be traced against many sources.
 struct Sample {
@@ -66,6 +219,56 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w) {
   s.py = (double)p.y;
   s.pz = (double)p.z;
   s.K = plane_consts(S, p);
+  return s;
+}
+
+// Record of a sample (layout documented at IACT_REC_WORDS).
+IACT_DEV void store_record(float* r, const Sample& s) {
+  double* d = (double*)r;
+  d[0] = s.px;
+  d[1] = s.py;
+  d[2] = s.pz;
+  r[6] = s.n.x;
+  r[7] = s.n.y;
+  r[8] = s.n.z;
+  r[9] = s.nrw;
+  r[10] = s.K.kt;
+  r[11] = s.K.kx;
+  r[12] = s.K.ky;
+  r[13] = s.p.x;
+  r[14] = s.p.y;
+  r[15] = s.p.z;
+}
+
+// Record as four 16-byte vector loads (every lane reads the same address:
+// one broadcast line).  Vector loads complete in order, so a load issued one
+// sample ahead is waited for on its own.
+struct RecRaw {
+  float4 a, b, c, d;
+};
+IACT_DEV RecRaw fetch_record(const float4* r) { return RecRaw{r[0], r[1], r[2], r[3]}; }
+IACT_DEV Sample unpack_record(const RecRaw& q) {
+  Sample s;
+  s.px = __hiloint2double(__float_as_int(q.a.y), __float_as_int(q.a.x));
+  s.py = __hiloint2double(__float_as_int(q.a.w), __float_as_int(q.a.z));
+  s.pz = __hiloint2double(__float_as_int(q.b.y), __float_as_int(q.b.x));
+  s.n = mk3(q.b.z, q.b.w, q.c.x);
+  s.nrw = q.c.y;
+  s.K = PlaneK{q.c.z, q.c.w, q.d.x};
+  s.p = mk3(q.d.y, q.d.z, q.d.w);
+  return s;
+}
+
+// Wave-uniform (scalar) load of a record.
+IACT_DEV Sample load_record(cptr<uint32_t> r) {
+  Sample s;
+  s.px = __hiloint2double((int)r[1], (int)r[0]);
+  s.py = __hiloint2double((int)r[3], (int)r[2]);
+  s.pz = __hiloint2double((int)r[5], (int)r[4]);
+  s.n = mk3(__uint_as_float(r[6]), __uint_as_float(r[7]), __uint_as_float(r[8]));
+  s.nrw = __uint_as_float(r[9]);
+  s.K = PlaneK{__uint_as_float(r[10]), __uint_as_float(r[11]), __uint_as_float(r[12])};
+  s.p = mk3(__uint_as_float(r[13]), __uint_as_float(r[14]), __uint_as_float(r[15]));
   return s;
 }
 

@@ -31,12 +31,16 @@ class LaunchConfig:
     """Launch shape handed to the kernels (see iactrace_amd.autotune)."""
 
     def __init__(self, block_threads=0, source_tile=0, blocks_per_cu=0, deterministic=False,
-                 accumulate=0):
+                 accumulate=0, chunk_block=0, block_order=0, layout=0, sample_group=0):
         self.block_threads = int(block_threads)
         self.source_tile = int(source_tile)
         self.blocks_per_cu = int(blocks_per_cu)
         self.deterministic = bool(deterministic)
         self.accumulate = int(accumulate)
+        self.chunk_block = int(chunk_block)
+        self.block_order = int(block_order)
+        self.layout = int(layout)
+        self.sample_group = int(sample_group)
 
     def abi(self):
         L = _lib.IactLaunch()
@@ -45,12 +49,18 @@ class LaunchConfig:
         L.blocks_per_cu = self.blocks_per_cu
         L.deterministic = int(self.deterministic)
         L.accumulate = self.accumulate
+        L.chunk_block = self.chunk_block
+        L.block_order = self.block_order
+        L.layout = self.layout
+        L.sample_group = self.sample_group
         return L
 
     def __repr__(self):
         return (f"LaunchConfig(block_threads={self.block_threads}, source_tile="
                 f"{self.source_tile}, blocks_per_cu={self.blocks_per_cu}, "
-                f"deterministic={self.deterministic}, accumulate={self.accumulate})")
+                f"deterministic={self.deterministic}, accumulate={self.accumulate}, "
+                f"chunk_block={self.chunk_block}, block_order={self.block_order}, "
+                f"layout={self.layout}, sample_group={self.sample_group})")
 
 
 def _stages(groups):

@@ -147,6 +147,20 @@ typedef struct iact_launch {
   int32_t accumulate;      /* pixel accumulation: 0 wave-merged (default),
                               1 per-lane run-length, 2 direct per-ray atomics;
                               deterministic mode always uses 0               */
+  int32_t chunk_block;     /* work order: sample chunks per block, inside
+                              which the source tile is the slower index;
+                              0 = one grid sweep of waves (default),
+                              -1 = all chunks (source tile slowest)          */
+  int32_t block_order;     /* 0 = XCD-major block numbering (default),
+                              1 = hardware block numbering                  */
+  int32_t layout;          /* image renders: 0 = automatic, 1 = lanes are
+                              the samples of one facet (wave-merged pixel
+                              sums; best when a wave's rays share pixels),
+                              2 = lanes are sources, samples read from a
+                              per-call world-space record table in the
+                              workspace (one LDS add per ray; best for many
+                              sources spread over the sensor)               */
+  int32_t sample_group;    /* layout 2: samples per work item (0 = 64)     */
 } iact_launch;
 
 /* Gradient outputs (fp64, overwritten).  NULL = not requested.               */

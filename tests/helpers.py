@@ -92,12 +92,19 @@ def grown_obstructions(ob, eps):
 def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
     """Rays (facet-major debug order) whose fate an fp32 rounding can flip:
     sensor coordinate within px_band pixels (or hex-norm units) of a binning
-    boundary, or a shadow/aperture decision that changes when obstructions and
-    stage>=1 apertures are grown or shrunk by ob_eps metres (float64 oracle).
+    boundary; a shadow/aperture decision that changes when obstructions and
+    stage>=1 apertures are grown or shrunk by ob_eps metres (float64 oracle);
+    or one that the reference's fp32 arithmetic (the ref32 oracle, no fused
+    multiply-adds) gets differently from float64 -- e.g. the discriminant
+    b^2 - 4ac of a ray grazing a cylinder, where an evaluation with a fused
+    multiply-add (as the kernels, and XLA where it contracts) may side with
+    float64 instead.
     """
     x, y, v, _ = O.render(sc, sources, values, source_type, "fp64", "debug")
     band = O.edge_margin(x, y, sc.sensor) < px_band
     alive = v != 0
+    _, _, v32, _ = O.render(sc, sources, values, source_type, "ref32", "debug")
+    band |= (v32 != 0) != alive
     import copy
     for eps in (ob_eps, -ob_eps):
         sc2 = copy.copy(sc)

@@ -143,6 +143,75 @@ def test_error_paths_without_gpu():
     assert rc == _lib.IACT_ERR_INVALID
 
 
+def test_batched_and_fit_error_paths_without_gpu():
+    L = _lib.lib()
+    sc = _valid_square_scene()
+    rc, msg = _status(L.iact_trace_batched_fwd(ctypes.byref(sc), None, 0, None, None, 0, None,
+                                               None))
+    assert rc == _lib.IACT_ERR_INVALID and "batch" in msg
+    B = _lib.IactBatch()
+    B.n_instances = 2
+    rc, msg = _status(L.iact_trace_batched_fwd(ctypes.byref(sc), ctypes.byref(B), 0, None, None,
+                                               0, None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "offsets" in msg
+    off = np.array([0, 3, 1], np.int64)
+    B.rotations, B.source_offsets = 8, 8
+    B.host_source_offsets = off.ctypes.data_as(ctypes.c_void_p)
+    rc, msg = _status(L.iact_trace_batched_fwd(ctypes.byref(sc), ctypes.byref(B), 0, None, None,
+                                               0, None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "decrease" in msg
+    off = np.array([0, 3, 5], np.int64)
+    B.host_source_offsets = off.ctypes.data_as(ctypes.c_void_p)
+    rc, msg = _status(L.iact_trace_batched_fwd(ctypes.byref(sc), ctypes.byref(B), 0, None, None,
+                                               0, None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "sources" in msg
+    B.sources, B.values = 8, 8
+    Lc = _lib.IactLaunch()
+    Lc.deterministic = 1
+    rc, msg = _status(L.iact_trace_batched_fwd(ctypes.byref(sc), ctypes.byref(B), 0,
+                                               ctypes.byref(Lc), None, 0, None, None))
+    assert rc == _lib.IACT_ERR_UNSUPPORTED and "deterministic" in msg
+    need = L.iact_batch_workspace_size(ctypes.byref(sc), ctypes.byref(B), None)
+    assert need >= 3 * 8 and need % 256 == 0
+    rc, msg = _status(L.iact_trace_batched_fwd(ctypes.byref(sc), ctypes.byref(B), 0, None,
+                                               ctypes.c_void_p(256), need - 1, None, None))
+    assert rc == _lib.IACT_ERR_WORKSPACE
+    rc, msg = _status(L.iact_trace_batched_bwd(ctypes.byref(sc), ctypes.byref(B), 3, None, None,
+                                               None, 0, None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "source type" in msg
+    # loss / optimiser helpers
+    rc, msg = _status(L.iact_l2_loss_grad(None, None, 4, 2, None, None, None))
+    assert rc == _lib.IACT_ERR_INVALID
+    assert L.iact_l2_loss_grad(None, None, 4, 0, None, None, None) == _lib.IACT_OK
+    rc, msg = _status(L.iact_adam_step(None, None, None, None, None, 4, 1e-3, 0.9, 0.999, 1e-8,
+                                       None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "step" in msg
+    rc, msg = _status(L.iact_adam_step(ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8),
+                                       ctypes.c_void_p(8), ctypes.c_void_p(8), 4, 1e-3, 1.5,
+                                       0.999, 1e-8, None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "betas" in msg
+    h = ctypes.c_void_p()
+    rc, msg = _status(L.iact_fit_graph_create(ctypes.byref(sc), None, 0, None, None, 0, None,
+                                              None, None, None, None, None, None, None, None,
+                                              1e-3, ctypes.byref(h)))
+    assert rc == _lib.IACT_ERR_INVALID and h.value is None
+    assert L.iact_graph_launch(None, None) == _lib.IACT_ERR_INVALID
+    assert L.iact_graph_destroy(None) == _lib.IACT_OK
+
+
+def test_pack_sources_and_c5_workload():
+    s, v, off = ia.pack_sources([np.zeros((2, 3)), np.zeros((0, 3)), np.ones((3, 3))],
+                                [np.ones(2), np.ones(0), np.ones(3)])
+    assert s.shape == (5, 3) and v.dtype == np.float32 and off.tolist() == [0, 2, 2, 5]
+    with pytest.raises(ValueError):
+        ia.pack_sources([np.zeros((2, 3))], [np.ones(3)])
+    bw = configs.batch_workload(64)
+    n = np.diff(bw.offsets)
+    assert len(n) == 64 and n.min() >= 256 and n.max() <= 16384
+    assert bw.n_rays == 875 * 1024 * int(bw.offsets[-1])
+    assert 4e11 < bw.n_rays < 6e11
+
+
 def test_error_text_is_thread_local_and_raised():
     with pytest.raises(_lib.IactError) as e:
         _lib.check(_lib.lib().iact_trace_fwd(None, None, None, 0, 0, 0, None, None, 0, None,

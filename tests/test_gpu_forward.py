@@ -24,10 +24,12 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 
-def _trace_and_compare(tel, src, vals, stype, label, rtol=1e-5):
+def _trace_and_compare(tel, src, vals, stype, label, rtol=1e-5, launch=None):
     print(f"\n[{label}] facets={sum(len(g) for g in tel.mirror_groups if g.optical_stage == 0)} "
-          f"samples={tel.mirror_groups[0].n_samples} sources={len(src)}", flush=True)
-    img = ia.render(tel, torch.as_tensor(src), torch.as_tensor(vals), stype).cpu().numpy()
+          f"samples={tel.mirror_groups[0].n_samples} sources={len(src)} launch={launch}",
+          flush=True)
+    img = ia.render(tel, torch.as_tensor(src), torch.as_tensor(vals), stype,
+                    launch=launch).cpu().numpy()
     xy, vk, pk = ia.render_debug(tel, torch.as_tensor(src), torch.as_tensor(vals), stype,
                                  return_pixels=True)
     xy, vk, pk = xy.cpu().numpy(), vk.cpu().numpy(), pk.cpu().numpy()
@@ -85,6 +87,63 @@ def test_c3_parabolic_hex_obstructions():
     tel, wl = _tel("C3", ns=6, m=8)
     img, _ = _trace_and_compare(tel, wl.sources, wl.values, "point", "C3")
     assert img.shape == (2048,)
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+def test_layouts_c2_many_sources(layout):
+    """Both image layouts (lanes = samples of a facet / lanes = sources) on
+    more sources than a wave holds, against the oracle."""
+    tel, wl = _tel("C2", ns=80, m=16, key=4)
+    _trace_and_compare(tel, wl.sources, wl.values, "point", f"C2 layout {layout}",
+                       launch=ia.LaunchConfig(layout=layout))
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+def test_layouts_c3_hex_obstructions(layout):
+    tel, wl = _tel("C3", ns=70, m=8, key=5)
+    _trace_and_compare(tel, wl.sources, wl.values, "point", f"C3 layout {layout}",
+                       launch=ia.LaunchConfig(layout=layout, sample_group=5))
+
+
+@pytest.mark.parametrize("kind", ["square", "hex"])
+def test_layouts_agree_on_splat_and_parallel(kind):
+    """The two layouts trace identical rays; only the order of the float
+    additions differs."""
+    if kind == "square":
+        tel, wl = _tel("C2", ns=8, m=64, key=6)
+        s = tel.sensors[0]
+        tel = tel.replace_sensor(ia.DifferentiableSquareSensor(s.position, s.rotation, s.width,
+                                                               s.height, s.bounds, 0.5, 2), 0)
+        src = configs.parallel_grid(10, 13, 0.2)
+        vals = np.linspace(0.5, 2.0, len(src)).astype(np.float32)
+        stype = "parallel"
+    else:
+        tel, wl = _tel("C3", ns=8, m=32, key=7)
+        s = tel.sensors[0]
+        tel = tel.replace_sensor(ia.DifferentiableHexagonalSensor(s.position, s.rotation,
+                                                                  s.hex_centers, 0.6, 1), 0)
+        src, vals = configs.point_source_field(150, 2.0, seed=11)
+        stype = "point"
+    a = ia.render(tel, torch.as_tensor(src), torch.as_tensor(vals), stype,
+                  launch=ia.LaunchConfig(layout=1)).reshape(-1)
+    b = ia.render(tel, torch.as_tensor(src), torch.as_tensor(vals), stype,
+                  launch=ia.LaunchConfig(layout=2, sample_group=33)).reshape(-1)
+    err = ((a - b).abs().max() / a.abs().max()).item()
+    print(f"[layouts {kind} splat] sum {a.sum().item():.6g} vs {b.sum().item():.6g}, "
+          f"max rel err {err:.2e}", flush=True)
+    assert a.abs().max().item() > 0
+    assert err < 2e-6
+
+
+def test_source_major_layout_rejects_stages_and_deterministic():
+    tel, wl = _tel("C4", ns=70, m=16)
+    with pytest.raises(ia._lib.IactError):
+        ia.render(tel, torch.as_tensor(wl.sources), torch.as_tensor(wl.values), "parallel",
+                  launch=ia.LaunchConfig(layout=2))
+    tel, wl = _tel("C2", ns=70, m=16)
+    with pytest.raises(ia._lib.IactError):
+        ia.render(tel, torch.as_tensor(wl.sources), torch.as_tensor(wl.values), "point",
+                  launch=ia.LaunchConfig(layout=2, deterministic=True))
 
 
 def test_c4_cassegrain_response_matrix():
