@@ -6,10 +6,11 @@ f = 15 m) x 4096 MC samples per facet x 4096 point sources at 1e10 m, square
 whole workload; sources, values and samples are resident in HBM before the
 timed region starts.
 
-Also reported: the roofline of the trace kernel (algorithmic flops per launch
-over its HIP-event duration, against MI355X's fp32 vector peak), HBM traffic
-from the committed rocprofv3 counter summary if present, and a CPU baseline:
-the oracle's C/OpenMP port timed on the host cores on a bounded source subsample.
+Also reported: the roofline of the trace (algorithmic flops per step over the
+step's HIP-event duration, against MI355X's fp32 vector peak), HBM traffic
+from the committed rocprofv3 counter summary if present, the C3
+forward+backward and C5 batched lines, and a CPU baseline: the oracle's
+C/OpenMP port timed on the host cores on a bounded source subsample.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -55,9 +56,9 @@ def time_steps(fn, steps, warmup):
 
 
 def traffic_from_profiles(name):
-    """HBM bytes per k_trace launch from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", f"k_trace_{name}_traffic.txt")
+    """HBM bytes per step from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE of the step's kernels, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", f"k_trace_sm_{name}_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -255,7 +256,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic,
-                     "kernel": "k_trace (fill + trace per step, HIP events)",
+                     "kernel": "k_trace_sm, timed per step with HIP events around the "
+                               "render call (k_fill_f32 + k_sample_records + k_trace_sm; "
+                               "the rocprofv3 summary gives each)",
                      "kernel_ms": kern_ms,
                      "flops_per_ray": {"fp32": f32, "fp64": f64},
                      "fp64_frac_of_fp64_peak": n_rays * f64 / (kern_ms * 1e-3) / PEAK_FP64},
