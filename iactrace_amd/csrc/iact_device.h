@@ -230,7 +230,7 @@ IACT_DEV double newton_t(const SurfT<double>& s, double ox, double oy, double oz
 // ---------------------------------------------------------------- obstructions
 // Each returns the ray parameter of the nearest hit or +inf
 // (intersections.py:44-226).  The shadow test only asks t < 1e10.
-IACT_DEV float hit_cylinder(f3 o, f3 d, const float* c) {
+IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
   const float eps = 1e-8f;
   const f3 p1 = mk3(c[0], c[1], c[2]);
   const f3 ax = mk3(c[3], c[4], c[5]);
@@ -257,7 +257,7 @@ IACT_DEV float hit_cylinder(f3 o, f3 d, const float* c) {
   return fminf(fminf(t1, t2), fminf(tb, tt));
 }
 
-IACT_DEV float hit_box(f3 o, f3 d, const float* b) {
+IACT_DEV float hit_box(f3 o, f3 d, cptr<float> b) {
   const float eps = 1e-8f;
   const float ix = frcp(d.x + eps), iy = frcp(d.y + eps), iz = frcp(d.z + eps);
   const float tx1 = (b[0] - o.x) * ix, tx2 = (b[3] - o.x) * ix;
@@ -269,7 +269,7 @@ IACT_DEV float hit_box(f3 o, f3 d, const float* b) {
   return hit ? (tmin > eps ? tmin : tmax) : __builtin_inff();
 }
 
-IACT_DEV float hit_sphere(f3 o, f3 d, const float* s) {
+IACT_DEV float hit_sphere(f3 o, f3 d, cptr<float> s) {
   const float eps = 1e-8f;
   const f3 oc = o - mk3(s[0], s[1], s[2]);
   const float a = dot(d, d), b = 2.f * dot(oc, d), c = dot(oc, oc) - s[3] * s[3];
@@ -283,7 +283,7 @@ IACT_DEV float hit_sphere(f3 o, f3 d, const float* s) {
 
 IACT_DEV float signf1(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
 
-IACT_DEV float hit_oriented_box(f3 o, f3 d, const float* q) {
+IACT_DEV float hit_oriented_box(f3 o, f3 d, cptr<float> q) {
   const float eps = 1e-8f;
   m3 R;
 #pragma unroll
@@ -303,7 +303,7 @@ IACT_DEV float hit_oriented_box(f3 o, f3 d, const float* q) {
   return (hit && tr > eps) ? tr : __builtin_inff();
 }
 
-IACT_DEV float hit_triangle(f3 o, f3 d, const float* v) {
+IACT_DEV float hit_triangle(f3 o, f3 d, cptr<float> v) {
   const float eps = 1e-8f;
   const f3 v0 = mk3(v[0], v[1], v[2]);
   const f3 e1 = mk3(v[3], v[4], v[5]) - v0;
@@ -324,33 +324,35 @@ IACT_DEV float hit_triangle(f3 o, f3 d, const float* v) {
 // 0 if any obstruction has t < 1e10 along the unbounded ray (render.py:21-41).
 // Once every lane of the wave is blocked the remaining primitives cannot
 // change any result, so the test returns early for the whole wave.
+// Obstruction rows are read through the constant address space: the
+// primitive index is wave-uniform, so they become scalar loads.
 IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool active) {
   const float big = 1e10f;
   bool blocked = !active;
   for (int i = 0; i < ob.n_cylinders; ++i) {
     if (__all(blocked)) break;
-    blocked |= hit_cylinder(o, d, ob.cylinders + 8 * i) < big;
+    blocked |= hit_cylinder(o, d, as_const(ob.cylinders) + 8 * i) < big;
   }
   for (int i = 0; i < ob.n_boxes; ++i) {
     if (__all(blocked)) break;
-    blocked |= hit_box(o, d, ob.boxes + 6 * i) < big;
+    blocked |= hit_box(o, d, as_const(ob.boxes) + 6 * i) < big;
   }
   for (int i = 0; i < ob.n_spheres; ++i) {
     if (__all(blocked)) break;
-    blocked |= hit_sphere(o, d, ob.spheres + 4 * i) < big;
+    blocked |= hit_sphere(o, d, as_const(ob.spheres) + 4 * i) < big;
   }
   for (int i = 0; i < ob.n_oriented_boxes; ++i) {
     if (__all(blocked)) break;
-    blocked |= hit_oriented_box(o, d, ob.oriented_boxes + 15 * i) < big;
+    blocked |= hit_oriented_box(o, d, as_const(ob.oriented_boxes) + 15 * i) < big;
   }
   for (int i = 0; i < ob.n_triangles; ++i) {
     if (__all(blocked)) break;
-    blocked |= hit_triangle(o, d, ob.triangles + 9 * i) < big;
+    blocked |= hit_triangle(o, d, as_const(ob.triangles) + 9 * i) < big;
   }
   return (blocked && active) ? 0.f : 1.f;
 }
 
-IACT_DEV bool has_obstructions(const iact_obstructions& ob) {
+__host__ __device__ inline bool has_obstructions(const iact_obstructions& ob) {
   return (ob.n_cylinders | ob.n_boxes | ob.n_spheres | ob.n_oriented_boxes | ob.n_triangles) != 0;
 }
 
@@ -463,6 +465,13 @@ IACT_DEV int hex_pixel(const DevSensor& S, const int32_t* table, float x, float 
   const int pid = hex_lookup(S, table, qi, ri);
   const float hd = hex_norm_axial(q - qi, r - ri);
   return (pid >= 0 && !(hd > S.edge_thr)) ? pid : -1;
+}
+
+// Add to a block's LDS image: relaxed, workgroup scope (the image is private
+// to the block; a distinct scope also keeps the compiler from merging it with
+// a global add into one flat atomic).
+IACT_DEV void lds_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---------------------------------------------------------------- wave ops

@@ -327,7 +327,7 @@ __global__ void k_sample_records(TraceArgs A, float* recs) {
 // source's rays on one or two pixels) and adds the run to the LDS image when
 // the pixel changes; splat taps are added one by one.  The per-ray arithmetic
 // is trace_ray's, as in the sample-major kernel.
-template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS>
+template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   extern __shared__ float smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
@@ -343,7 +343,16 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   }
   __syncthreads();
   const int32_t* table = tab_lds;
-  float* const acc = A.lds_image ? img : A.out;
+  // pixel adds go to the LDS image or, when it does not fit, to global
+  // memory -- two separate pointers, so each add is a plain LDS or global
+  // atomic (a pointer that may be either would make it a flat atomic, which
+  // also stalls the record prefetch)
+  auto add_px = [&](int pix, float v) {
+    if (A.lds_image)
+      lds_add(img + pix, v);
+    else
+      unsafeAtomicAdd(A.out + pix, v);
+  };
   const long long total = (long long)A.nf * A.M;
   for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
        item += (long long)gridDim.x * wpb) {
@@ -358,26 +367,19 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const long long j0 = (long long)g * A.sample_group;
     const long long j1 = min(j0 + A.sample_group, total);
     const float4* recs = (const float4*)A.recs;
-    RecRaw cur_r = fetch_record(recs + 4 * j0);
     int run_pix = -1;
     float run_v = 0.f;
-    for (long long j = j0; j < j1; ++j) {
-      const long long jn = j + 1 < j1 ? j + 1 : j;
-      const RecRaw nxt_r = fetch_record(recs + 4 * jn);
-      const Sample smp = unpack_record(cur_r);
+    auto trace_one = [&](const RecRaw& rec) {
+      const Sample smp = unpack_record(rec);
       float x, y, v;
-      const bool ok = trace_ray<SRC, STAGES, AXIS>(A, smp, active, sx, sy, sz, value, x, y, v);
+      const bool ok =
+          trace_ray<SRC, STAGES, AXIS, OBST>(A, smp, active, sx, sy, sz, value, x, y, v);
       if (!SPLAT) {
         // run of equal pixels per lane (consecutive samples of a facet put a
         // source's rays on the same one or two pixels), added when it ends
         const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
         if (pix != run_pix) {
-          if (run_pix >= 0 && run_v != 0.f) {
-            if (A.lds_image)
-              atomicAdd(acc + run_pix, run_v);
-            else
-              unsafeAtomicAdd(acc + run_pix, run_v);
-          }
+          if (run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
           run_pix = pix;
           run_v = 0.f;
         }
@@ -385,22 +387,21 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       } else {
         splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
           const float vw = v * w;
-          if (pix >= 0 && vw != 0.f) {
-            if (A.lds_image)
-              atomicAdd(acc + pix, vw);
-            else
-              unsafeAtomicAdd(acc + pix, vw);
-          }
+          if (pix >= 0 && vw != 0.f) add_px(pix, vw);
         });
       }
-      cur_r = nxt_r;
+    };
+    // two record buffers used alternately (no register copies between
+    // iterations); each prefetches the record after the one being traced
+    RecRaw ra = fetch_record(recs + 4 * j0), rb;
+    for (long long j = j0; j < j1; j += 2) {
+      rb = fetch_record(recs + 4 * (j + 1 < j1 ? j + 1 : j));
+      trace_one(ra);
+      if (j + 1 >= j1) break;
+      ra = fetch_record(recs + 4 * (j + 2 < j1 ? j + 2 : j + 1));
+      trace_one(rb);
     }
-    if (!SPLAT && run_pix >= 0 && run_v != 0.f) {
-      if (A.lds_image)
-        atomicAdd(acc + run_pix, run_v);
-      else
-        unsafeAtomicAdd(acc + run_pix, run_v);
-    }
+    if (!SPLAT && run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
   }
   if (A.lds_image) {
     __syncthreads();
@@ -519,8 +520,11 @@ static TraceKernel pick_trace_t(const TraceArgs& A) {
 // layout only for those)
 template <int SRC, int KIND, int SPLAT>
 static TraceKernel pick_sm_t(const TraceArgs& A) {
-  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true>
-                  : k_trace_sm<SRC, KIND, SPLAT, false, false>;
+  if (has_obstructions(A.ob))
+    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1>
+                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1>;
+  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0>
+                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0>;
 }
 
 template <int SRC>

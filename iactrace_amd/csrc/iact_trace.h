@@ -368,7 +368,8 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
 // Returns false when the ray misses the plane (the reference's 1e10
 // sentinel) or the lane is inactive; v is 0 for inactive lanes.
-template <int SRC, bool STAGES, bool AXIS>
+// OBST: 0 = the scene has no obstructions, 1 = it has, 2 = test at run time.
+template <int SRC, bool STAGES, bool AXIS, int OBST = 2>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, double sx, double sy,
                         double sz, float value, float& x, float& y, float& v) {
   f3 d;
@@ -383,7 +384,8 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, double
   const float c = dot(d, s.n);
   f3 r = fma3(-2.f * c, s.n, d);
   v = value * c * s.nrw;
-  if (has_obstructions(A.ob)) v *= shadow_factor(A.ob, s.p, -d, active);
+  if (OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))
+    v *= shadow_factor(A.ob, s.p, -d, active);
   PlaneK K = s.K;
   if (STAGES && A.n_stages > 0) {
     f3 o = s.p;
