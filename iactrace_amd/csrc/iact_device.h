@@ -242,11 +242,19 @@ IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
   const float b = 2.f * dot(ocp, rdp);
   const float cc = dot(ocp, ocp) - r * r;
   const float disc = b * b - 4.f * a * cc;
+  const float inf = __builtin_inff();
+  // Early out.  -disc / (4a) is the squared distance of the ray's line from
+  // the axis minus r^2; when it exceeds ~1e-4 r (the line passes more than
+  // ~5e-5 m outside the surface) plus a bound on the rounding of disc, no
+  // side or cap hit is possible.  The per-lane result is the miss for such a
+  // lane whatever its wave, and the rest of the test is skipped when it
+  // holds for the whole wave.
+  const bool far = -disc > fmaf(4.f * a, 1e-4f * r, 1e-6f * fmaf(4.f * a, fabsf(cc), b * b));
+  if (__all(far)) return inf;
   const float sq = fsqrt(fmaxf(disc, 0.f));
   const float ia = frcp(2.f * a + eps);
   float t1 = (-b - sq) * ia, t2 = (-b + sq) * ia;
   const float y1 = oca + t1 * rda, y2 = oca + t2 * rda;
-  const float inf = __builtin_inff();
   t1 = (t1 > eps && y1 >= 0.f && y1 <= h && disc >= 0.f) ? t1 : inf;
   t2 = (t2 > eps && y2 >= 0.f && y2 <= h && disc >= 0.f) ? t2 : inf;
   const float ir = frcp(rda + eps);
@@ -254,7 +262,7 @@ IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
   const f3 pb = ocp + tb * rdp, pt = ocp + tt * rdp;
   tb = (tb > eps && dot(pb, pb) <= r * r) ? tb : inf;
   tt = (tt > eps && dot(pt, pt) <= r * r) ? tt : inf;
-  return fminf(fminf(t1, t2), fminf(tb, tt));
+  return far ? inf : fminf(fminf(t1, t2), fminf(tb, tt));
 }
 
 IACT_DEV float hit_box(f3 o, f3 d, cptr<float> b) {

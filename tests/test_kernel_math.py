@@ -1,0 +1,55 @@
+"""CPU restatements of numerical shortcuts the HIP kernels take, checked to
+be conservative against the oracle (the kernels themselves are checked on the
+GPU by the parity tests)."""
+import numpy as np
+
+import oracle.iact_oracle as O
+
+
+def _far(o, d, p1, ax, r):
+    """The cylinder test's early out (iact_device.h, hit_cylinder) in fp32:
+    the ray's line is farther than ~5e-5 m outside the surface, with a bound
+    on the rounding of the discriminant."""
+    f = np.float32
+    oc = (o - p1).astype(f)
+    oca = oc @ ax
+    rda = d @ ax
+    ocp = oc - oca[:, None] * ax
+    rdp = d - rda[:, None] * ax
+    a = np.sum(rdp * rdp, 1)
+    b = f(2) * np.sum(ocp * rdp, 1)
+    cc = np.sum(ocp * ocp, 1) - r * r
+    disc = b * b - f(4) * a * cc
+    margin = f(4) * a * (f(1e-4) * r) + f(1e-6) * (f(4) * a * np.abs(cc) + b * b)
+    return -disc > margin
+
+
+def test_cylinder_early_out_never_skips_a_hit():
+    """Rays aimed at random points close to a long thin cylinder (the
+    telescope-support geometry): a ray the early out skips is a miss of the
+    reference's fp32 test, and of the fp64 test with the radius grown by
+    2e-5 m (so it is outside the shadow band)."""
+    rng = np.random.default_rng(0)
+    f = np.float32
+    p1 = np.array([10.0, 10.0, 1.4])
+    p2 = np.array([1.0, 1.0, 36.25])
+    r = 0.1
+    args32 = O.cylinder_consts(p1, p2, r, f)
+    args64g = O.cylinder_consts(p1, p2, r + 2e-5, np.float64)
+    n = 200000
+    # origins on a dish-like disc, targets near the cylinder axis
+    o = np.stack([rng.uniform(-12, 12, n), rng.uniform(-12, 12, n), rng.uniform(0, 3, n)], 1)
+    s = rng.uniform(0, 1, n)[:, None]
+    axis_pt = p1 + s * (p2 - p1)
+    jitter = rng.normal(0, 1, (n, 3)) * rng.choice([0.05, 0.1, 0.15, 0.3], n)[:, None]
+    tgt = axis_pt + jitter
+    d = tgt - o
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    o32, d32 = o.astype(f), d.astype(f)
+    far = _far(o32, d32, args32[0], args32[1], args32[3])
+    hit32 = np.isfinite(O.intersect_cylinder(o32, d32, *args32))
+    hit64g = np.isfinite(O.intersect_cylinder(o, d, *args64g))
+    print(f"rays {n}: far {far.mean():.3f}, hits {hit32.mean():.3f}")
+    assert 0.2 < far.mean() < 0.95 and hit32.mean() > 0.02
+    assert not np.any(far & hit32)
+    assert not np.any(far & hit64g)
