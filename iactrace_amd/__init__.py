@@ -7,6 +7,7 @@ Drop-in surface of the reference (iactrace/__init__.py:1-19): ``Telescope``,
 batched / differentiable extensions.  Every render runs in the HIP kernels of
 libiactrace_amd.so; there is no other backend.
 """
+from .autotune import autotune
 from .batched import AlignmentFit, pack_sources, render_batched
 from .geometry import (AsphericSurface, DiskAperture, PolygonAperture, euler_to_matrix,
                        look_at_euler, look_at_rotation)
@@ -33,5 +34,5 @@ __all__ = [
     "CylinderGroup", "Box", "BoxGroup", "Sphere", "SphereGroup", "OrientedBox",
     "OrientedBoxGroup", "Triangle", "TriangleGroup", "group_obstructions", "euler_to_matrix",
     "look_at_euler", "look_at_rotation", "split_key", "render_batched", "pack_sources",
-    "AlignmentFit",
+    "AlignmentFit", "autotune",
 ]

@@ -148,6 +148,9 @@ _SIGNATURES = {
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p, c_float,
                                         POINTER(c_void_p)]),
+    "iact_autotune": (c_int32, [POINTER(IactScene), c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                POINTER(IactLaunch), c_void_p, c_size_t, c_void_p, c_int32,
+                                POINTER(IactLaunch), POINTER(c_float), c_void_p]),
     "iact_graph_launch": (c_int32, [c_void_p, c_void_p]),
     "iact_graph_destroy": (c_int32, [c_void_p]),
 }

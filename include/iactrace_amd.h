@@ -281,6 +281,21 @@ int32_t iact_adam_step(float* params, const double* grads, float* exp_avg, float
                        int32_t* step, int64_t n, float lr, float beta1, float beta2,
                        float eps, const int32_t* mask3, void* stream);
 
+/* ---- launch-shape autotuner (no reference counterpart: XLA chooses its own
+ * launch shapes) ------------------------------------------------------------
+ * Times candidate iact_launch settings -- image layout, block size, blocks
+ * per CU, source tile / sample group -- for one IMAGE or RESPONSE render of
+ * the given inputs (`reps` timed renders each, after a warm-up) and writes
+ * the fastest to *best (and its time to *best_ms, nullable).  `base` supplies
+ * the fields that are kept (deterministic, accumulate; nullable).  Candidates
+ * needing more workspace than given are skipped.  Synchronises `stream`;
+ * `out` receives the last render.                                            */
+int32_t iact_autotune(const iact_scene* scene, const double* sources, const float* values,
+                      int32_t n_sources, int32_t source_type, int32_t mode,
+                      const iact_launch* base, void* workspace, size_t workspace_bytes,
+                      float* out, int32_t reps, iact_launch* best, float* best_ms,
+                      void* stream);
+
 /* ---- HIP graphs ---------------------------------------------------------
  * Capture a fit iteration (batched forward, loss gradient, batched backward,
  * Adam) once and replay it.  The graph handle owns its executable graph; all

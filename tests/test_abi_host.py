@@ -199,6 +199,27 @@ def test_batched_and_fit_error_paths_without_gpu():
     assert L.iact_graph_destroy(None) == _lib.IACT_OK
 
 
+def test_autotune_error_paths_and_key():
+    L = _lib.lib()
+    sc = _valid_square_scene()
+    rc, msg = _status(L.iact_autotune(ctypes.byref(sc), None, None, 0, 0, 0, None, None, 0, None,
+                                      1, None, None, None))
+    assert rc == _lib.IACT_ERR_INVALID and "result" in msg
+    best = _lib.IactLaunch()
+    rc, msg = _status(L.iact_autotune(ctypes.byref(sc), None, None, 0, 0, _lib.MODE_DEBUG, None,
+                                      None, 0, ctypes.c_void_p(8), 1, ctypes.byref(best), None,
+                                      None))
+    assert rc == _lib.IACT_ERR_INVALID and "image and response" in msg
+    from iactrace_amd.autotune import tuning_key
+    wl = configs.workload("C2", n_sources=8, n_samples=4)
+    from iactrace_amd.mirrors import group_mirrors
+    tel = ia.Telescope(group_mirrors(wl.mirrors), [], [wl.sensor])
+    k1 = tuning_key(tel, 3000, arch="gfx950:256cu")
+    k2 = tuning_key(tel, 4000, arch="gfx950:256cu")
+    k3 = tuning_key(tel, 5000, arch="gfx950:256cu")
+    assert k1 == k2 != k3 and '"sources": 4096' in k2
+
+
 def test_pack_sources_and_c5_workload():
     s, v, off = ia.pack_sources([np.zeros((2, 3)), np.zeros((0, 3)), np.ones((3, 3))],
                                 [np.ones(2), np.ones(0), np.ones(3)])
