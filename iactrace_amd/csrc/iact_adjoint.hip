@@ -167,11 +167,97 @@ IACT_DEV float splat_adjoint(const DevSensor& S, const int32_t* table, const flo
   }
 }
 
+// Stages >= 1 whose reflections the adjoint follows (the host rejects more).
+#define IACT_ADJ_MAX_STAGES 2
+
+// Add per-lane fp64 values into table[key] for lanes with key >= 0: lanes
+// sharing a key are summed across the wave first (one atomic per key).
+IACT_DEV void wave_add_keyed(double* table, int key, double v) {
+  int k = key;
+  for (int round = 0; round < 64; ++round) {
+    const unsigned long long live = __ballot(k >= 0);
+    if (live == 0ull) break;
+    const int leader = __ffsll((long long)live) - 1;
+    const int kl = __shfl(k, leader);
+    const bool mine = k == kl;
+    const double tot = wave_sum_d(mine ? v : 0.0);
+    if (lane_id() == leader) unsafeAtomicAdd(table + kl, tot);
+    if (mine) k = -1;
+  }
+}
+
+// Adjoint of one stage>=1 reflection (stage_forward).  In: the record and the
+// adjoints (go, gr, gv) of the outgoing origin, direction and value; out, in
+// place: the adjoints of the incoming ones.  The hit is the converged root of
+// F(t) = ol.z + t dl.z - S(ol.x + t dl.x, ol.y + t dl.y) in the mirror's
+// frame, so by the implicit function theorem dt = -(m.dol + t m.ddl -
+// dS/da da) / (m.dl) with m = (-Sx, -Sy, 1); the hit point is (x, y, S(x, y))
+// and the normal m / |m|, whose change with (x, y) needs the sag's Hessian.
+// Gradients of the hit surface's aspheric coefficients go to g_asph.
+IACT_DEV void stage_adjoint(const AdjArgs& B, const StageRec& rec, f3& go, f3& gr, float& gv,
+                            bool want_asph) {
+  const TraceArgs& A = B.T;
+  const f3 zero = mk3(0.f, 0.f, 0.f);
+  const bool live = rec.mi >= 0 && rec.hs != 0.f;
+  // a stage that zeroes the value zeroes every adjoint downstream of it
+  const float* M = A.mirrors + (size_t)IACT_MIRROR_STRIDE * (live ? rec.mi : 0);
+  const m3 R = mirror_rotation(M);
+  const f3 pos = mk3(M[0], M[1], M[2]);
+  const int si = (int)M[38];
+  const SurfT<float> su = load_surface<float>(A.surfaces[si]);
+  const float x0 = M[15], y0 = M[16];
+  const float z0 = sag_raw<float>(su, x0, y0);
+  const SurfD D = surface_derivs(su, rec.xh, rec.yh, x0, y0, z0);
+  const f3 m = mk3(-D.Sx, -D.Sy, 1.f);
+  const float imn = frsq(dot(m, m));
+  const f3 nl = imn * m;
+  const f3 bn = mul(R, nl);
+  const float c2 = rec.c2;
+  // value: v' = v hs |c2|; direction: r' = r - 2 c2 bn, c2 = r.bn
+  const float sgn = c2 > 0.f ? 1.f : (c2 < 0.f ? -1.f : 0.f);
+  const float gc2 = fmaf(gv * rec.v, sgn, -2.f * dot(bn, gr));
+  const f3 gbn = fma3(gc2, rec.r, (-2.f * c2) * gr);
+  f3 gr_in = fma3(gc2, bn, gr);
+  const float gv_in = gv * fabsf(c2);
+  // hit point and normal in the mirror frame
+  const f3 gpl = mulT(R, go);
+  const f3 gnl = mulT(R, gbn);
+  const f3 gm = imn * fma3(-dot(nl, gnl), nl, gnl);
+  const float gSx = -gm.x, gSy = -gm.y, gS = gpl.z;
+  const float gxh = gpl.x + gS * D.Sx + gSx * D.Sxx + gSy * D.Sxy;
+  const float gyh = gpl.y + gS * D.Sy + gSx * D.Sxy + gSy * D.Syy;
+  const f3 dl = mulT(R, rec.r);
+  const float md = dot(m, dl);
+  const float k = (gxh * dl.x + gyh * dl.y) * frcp(md);
+  const f3 gol = mk3(gxh - k * m.x, gyh - k * m.y, -k * m.z);
+  const f3 go_in = mul(R, gol);
+  gr_in = fma3(rec.t, mul(R, gol), gr_in);  // gdl = t gol
+  if (want_asph) {
+    const float r20 = fmaf(x0, x0, y0 * y0);
+    const float r4 = D.r2 * D.r2, r40 = r20 * r20;
+    float pw = r4, pw0 = r40, q = 1.f;
+#pragma unroll
+    for (int i = 0; i < IACT_MAX_ASPHERIC; ++i) {
+      const bool use = live && i < su.na;
+      const float dS = pw - pw0;                      // d S / d a_i
+      const float dg = (float)(2 * (i + 1)) * D.r2 * q;  // d (dz/dr2) / d a_i
+      const float ga = fmaf(gS + k, dS, 2.f * dg * (gSx * D.X + gSy * D.Y));
+      wave_add_keyed(B.g_asph, use ? IACT_MAX_ASPHERIC * si + i : -1, (double)ga);
+      pw *= r4;
+      pw0 *= r40;
+      q *= r4;
+    }
+  }
+  go = live ? go_in : zero;
+  gr = live ? gr_in : zero;
+  gv = live ? gv_in : 0.f;
+}
+
 // BATCH: many instances (TraceArgs batch fields); the LDS holds the image
 // gradient of the instance the block is working on and is reloaded when the
 // block moves on to the next instance; only rotation gradients, per
 // instance, are produced.
-template <int SRC, int KIND, int SPLAT, bool BATCH>
+template <int SRC, int KIND, int SPLAT, bool BATCH, bool STAGES>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   const TraceArgs& A = B.T;
   extern __shared__ float smem[];
@@ -187,6 +273,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   const bool want_src = !BATCH && B.g_src != nullptr, want_val = !BATCH && B.g_val != nullptr;
   const bool want_asph = !BATCH && B.g_asph != nullptr && B.surface_index != nullptr &&
                          B.offsets != nullptr;
+  const bool want_stage_asph = STAGES && B.g_asph != nullptr;
   const long long n_items = total_items(A);
   const long long n_bitems = (n_items + wpb - 1) / wpb;
   int cur = 0;  // instance whose image gradient the LDS holds
@@ -241,26 +328,47 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       const float shadow =
           has_obstructions(A.ob) ? shadow_factor(A.ob, smp.p, -d, active) : 1.f;
       const float vunit = active ? c * smp.nrw * shadow : 0.f;  // ray value / source value
+      // stages >= 1 (render.py:44-79), recorded for their adjoint
+      f3 of = smp.p, rf = r;
+      float vf = value * vunit;
+      StageRec rec[IACT_ADJ_MAX_STAGES];
+      PlaneK K = smp.K;
+      if (STAGES) {
+#pragma unroll
+        for (int k = 0; k < IACT_ADJ_MAX_STAGES; ++k)
+          if (k < A.n_stages) stage_forward<true>(A, A.stages[k], of, rf, vf, active, rec[k]);
+        K = plane_consts(A.S, of);
+      }
       float x, y;
-      const bool ok = plane_xy<false>(A.S, smp.K, r, x, y) && active;
+      const bool ok = plane_xy<false>(A.S, K, rf, x, y) && active;
       // ---- image adjoint -> ray adjoints
       float gv, gx = 0.f, gy = 0.f;
       if (!SPLAT) {
         const int pix = ok ? hard_pixel<KIND>(A.S, tab, x, y) : -1;
         gv = pix >= 0 ? G[pix] : 0.f;
       } else {
-        gv = splat_adjoint<KIND>(A.S, tab, G, x, y, ok, value * vunit, gx, gy);
+        gv = splat_adjoint<KIND>(A.S, tab, G, x, y, ok, vf, gx, gy);
       }
-      // value = value_s * c * nrw * shadow
+      // the sensor-plane hit depends on the last origin and direction
+      f3 go = mk3(0.f, 0.f, 0.f), gr = mk3(0.f, 0.f, 0.f);
+      if (SPLAT) {
+        const float t = K.kt * frcp(dot(rf, A.S.nrm));
+        plane_adjoint(A.S, rf, t, gx, gy, go, gr);
+      }
+      if (STAGES) {
+#pragma unroll
+        for (int k = IACT_ADJ_MAX_STAGES - 1; k >= 0; --k)
+          if (k < A.n_stages) stage_adjoint(B, rec[k], go, gr, gv, want_stage_asph);
+      }
+      // now (go, gr, gv) are the adjoints of the primary leg's origin (the
+      // sample), reflected direction and value = value_s * c * nrw * shadow
+      Ap = Ap + go;
       const float dc = gv * value * smp.nrw * shadow;
       Anrw = fmaf(gv * value * shadow, c, Anrw);
       f3 gd = dc * smp.n;
       An = fma3(dc, d, An);
-      if (SPLAT) {
-        // x, y depend on the origin (the sample) and on r = d - 2 (d.n) n
-        const float t = smp.K.kt * frcp(dot(r, A.S.nrm));
-        f3 gr = mk3(0.f, 0.f, 0.f);
-        plane_adjoint(A.S, r, t, gx, gy, Ap, gr);
+      if (SPLAT || STAGES) {
+        // r = d - 2 (d.n) n
         const float ng = dot(smp.n, gr);
         gd = gd + fma3(-2.f * ng, smp.n, gr);
         An = An + fma3(-2.f * ng, d, (-2.f * c) * gr);
@@ -375,7 +483,9 @@ using AdjKernel = void (*)(AdjArgs);
 
 template <int SRC, int KIND, int SPLAT>
 static AdjKernel pick_adj_t(const AdjArgs& B) {
-  return B.T.item_prefix ? k_trace_adj<SRC, KIND, SPLAT, true> : k_trace_adj<SRC, KIND, SPLAT, false>;
+  if (B.T.item_prefix) return k_trace_adj<SRC, KIND, SPLAT, true, false>;
+  return B.T.n_stages > 0 ? k_trace_adj<SRC, KIND, SPLAT, false, true>
+                          : k_trace_adj<SRC, KIND, SPLAT, false, false>;
 }
 
 template <int SRC>
@@ -424,10 +534,9 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL)
     return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
   const iact_scene& sc = *scene;
-  if (sc.n_stages > 0)
-    return set_error(IACT_ERR_UNSUPPORTED,
-                     "the adjoint kernel covers primary-only scenes; stage>=1 mirrors need the "
-                     "tangent kernel");
+  if (sc.n_stages > IACT_ADJ_MAX_STAGES)
+    return set_error(IACT_ERR_UNSUPPORTED, "the adjoint follows at most %d stages >= 1 (%d given)",
+                     IACT_ADJ_MAX_STAGES, sc.n_stages);
   if (grads->aspheric && (!sc.facets.surface_index || !sc.facets.offsets))
     return set_error(IACT_ERR_INVALID, "aspheric gradients need facet surface_index/offsets");
   hipStream_t st = (hipStream_t)stream;

@@ -181,6 +181,41 @@ IACT_DEV void surface_point_normal(const SurfT<float>& s, float x, float y, floa
   n = mk3(-zx * inv, -zy * inv, inv);
 }
 
+// S(x, y) = sag(x + x0, y + y0) - z0 with its gradient and Hessian at a
+// local point (fp32; used by the adjoint of stage>=1 hits).  The gradient is
+// the one surface_point_normal uses; the second derivative of the sag in r2
+// is c K / (4 (1 - K r2)^(3/2)) + sum_i a_i p_i (p_i - 1) r2^(p_i - 2),
+// p_i = 2(i+1).
+struct SurfD {
+  float S, Sx, Sy, Sxx, Sxy, Syy;
+  float X, Y, r2;
+};
+IACT_DEV SurfD surface_derivs(const SurfT<float>& s, float x, float y, float x0, float y0,
+                              float z0) {
+  SurfD D;
+  D.X = x + x0;
+  D.Y = y + y0;
+  D.r2 = D.X * D.X + D.Y * D.Y;
+  float z, g;
+  sag_and_dr2<float>(s, D.r2, z, g);
+  const float sq = fsqrt(1.f - s.K * D.r2);
+  float gp = s.c * s.K * 0.25f * frcp(sq * sq * sq);
+  float q = 1.f;  // r2^(p_i - 2) = r2^(2i)
+#pragma unroll
+  for (int i = 0; i < IACT_MAX_ASPHERIC; ++i) {
+    const float p = (float)(2 * (i + 1));
+    if (i < s.na) gp = fmaf(s.a[i] * p * (p - 1.f), q, gp);
+    q *= D.r2 * D.r2;
+  }
+  D.S = z - z0;
+  D.Sx = 2.f * D.X * g;
+  D.Sy = 2.f * D.Y * g;
+  D.Sxx = fmaf(4.f * D.X * D.X, gp, 2.f * g);
+  D.Sxy = 4.f * D.X * D.Y * gp;
+  D.Syy = fmaf(4.f * D.Y * D.Y, gp, 2.f * g);
+  return D;
+}
+
 // Closed-form conic root (intersections.py:229-285), float64.
 IACT_DEV double intersect_conic(double ox, double oy, double oz, double dx, double dy, double dz,
                                 double c, double k) {

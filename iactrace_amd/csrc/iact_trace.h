@@ -337,13 +337,28 @@ IACT_DEV m3 mirror_rotation(const float* M) {
   return R;
 }
 
+// What the adjoint needs to know about one stage>=1 reflection of a ray.
+struct StageRec {
+  f3 o, r;        // ray entering the stage
+  float v;        // value entering the stage
+  float t;        // hit parameter along the local ray
+  float xh, yh;   // local hit (x, y)
+  float c2;       // cos between the ray and the hit normal
+  float hs;       // hit * shadow (0 or 1)
+  int mi;         // mirror row, -1 when no mirror was hit
+};
+
 // _reflect_at_stage (render.py:44-79): nearest hit over the stage's mirrors
 // (first minimum wins, table is ordered by group then index), reflect, and
-// value *= hit * shadow * |cos|.  A miss keeps point 0 / normal 0.
-IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, float& v,
-                               bool active) {
+// value *= hit * shadow * |cos|.  A miss keeps point 0 / normal 0.  Fills rec
+// when REC.
+template <bool REC>
+IACT_DEV void stage_forward(const TraceArgs& A, int stage, f3& o, f3& d, float& v, bool active,
+                            StageRec& rec) {
   float best = __builtin_inff();
   f3 bp = mk3(0.f, 0.f, 0.f), bn = mk3(0.f, 0.f, 0.f);
+  int bi = -1;
+  float bx = 0.f, by = 0.f;
   for (int i = 0; i < A.n_mirrors; ++i) {
     const float* M = A.mirrors + (size_t)IACT_MIRROR_STRIDE * i;
     if ((int)M[36] != stage) continue;
@@ -354,15 +369,37 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
       best = h.t;
       bp = mul(R, h.pl) + pos;
       bn = mul(R, h.nl);
+      if (REC) {
+        bi = i;
+        bx = h.pl.x;
+        by = h.pl.y;
+      }
     }
   }
   const float c = dot(d, bn);
   const f3 r = fma3(-2.f * c, bn, d);
   const float hit = best < 1e10f ? 1.f : 0.f;
   const float shadow = has_obstructions(A.ob) ? shadow_factor(A.ob, o, d, active) : 1.f;
+  if (REC) {
+    rec.o = o;
+    rec.r = d;
+    rec.v = v;
+    rec.t = best;
+    rec.xh = bx;
+    rec.yh = by;
+    rec.c2 = c;
+    rec.hs = hit * shadow;
+    rec.mi = hit > 0.f ? bi : -1;
+  }
   v = v * hit * shadow * fabsf(c);
   o = bp;
   d = r;
+}
+
+IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, float& v,
+                               bool active) {
+  StageRec unused;
+  stage_forward<false>(A, stage, o, d, v, active, unused);
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).

@@ -234,3 +234,62 @@ def test_grad_vs_central_differences_total_flux_hard():
     tel = tel.replace_sensor(big, 0)
     _fd_check("fd-flux", tel, wl.sources, wl.values, "point", np.ones(big.n_pixels),
               [0, 17, 200, 381], 1e-2, 2e-3)
+
+
+def _cassegrain_tel(splat, m=512):
+    """C4's two-mirror telescope with aspheric terms on both mirrors."""
+    mirrors, sensor = configs.cassegrain()
+    mirrors[0].surface.aspheric = torch.tensor([3e-4, -2e-6, 1e-8, -5e-11])
+    mirrors[1].surface.aspheric = torch.tensor([2e-3, -4e-5])
+    if splat:
+        sensor = ia.DifferentiableSquareSensor(sensor.position, sensor.rotation, sensor.width,
+                                               sensor.height, sensor.bounds, sigma=0.7,
+                                               kernel_size=2)
+    groups = [g.to(DEV) for g in ia.group_mirrors(mirrors)]
+    groups[0] = ia.MCIntegrator(m).sample_group(groups[0], 13)
+    return ia.Telescope(groups, [], [sensor])
+
+
+@pytest.mark.parametrize("splat", [False, True])
+def test_grad_through_stage_two(splat):
+    """Gradients through the secondary mirror of a Cassegrain (hit by the
+    implicit function theorem on the Newton root, normal via the sag's
+    Hessian): primary pose, both mirrors' aspheric coefficients, source
+    directions and values, against the float64 twin (whose Newton iterations
+    are differentiated step by step)."""
+    tel = _cassegrain_tel(splat)
+    src = configs.parallel_grid(3, 2, 0.05)
+    vals = np.array([1.0, 0.7, 1.3, 0.9, 1.1, 0.8], np.float32)
+    W = _smooth_weights(tel.sensors[0])
+    # kernel
+    g0, g1 = tel.mirror_groups
+    rot = g0.rotations.detach().clone().requires_grad_(True)
+    pos = g0.positions.detach().clone().requires_grad_(True)
+    a0 = g0.aspheric.detach().clone().requires_grad_(True)
+    a1 = g1.aspheric.detach().clone().requires_grad_(True)
+    t2 = ia.Telescope([g0.replace(rotations=rot, positions=pos, aspheric=a0),
+                       g1.replace(aspheric=a1)], [], tel.sensors)
+    s = torch.tensor(src, dtype=torch.float64, device=DEV, requires_grad=True)
+    v = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+    img = ia.render(t2, s, v, "parallel")
+    (img.reshape(-1).double() * torch.as_tensor(W, device=DEV)).sum().backward()
+    gk = {"rot": rot.grad.double().cpu().numpy(), "pos": pos.grad.double().cpu().numpy(),
+          "asph": a0.grad.double().cpu().numpy(), "asph2": a1.grad.double().cpu().numpy(),
+          "src": s.grad.double().cpu().numpy(), "val": v.grad.double().cpu().numpy()}
+    # twin
+    sc = oracle_scene(tel)
+    trot = torch.tensor(sc.rotations, dtype=torch.float64, requires_grad=True)
+    tpos = torch.tensor(sc.positions, dtype=torch.float64, requires_grad=True)
+    ta0 = torch.tensor(g0.aspheric.double().numpy(), requires_grad=True)
+    ta1 = torch.tensor(g1.aspheric.double().numpy(), requires_grad=True)
+    ts = torch.tensor(src, dtype=torch.float64, requires_grad=True)
+    tv = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    timg = TW.render_image(sc, ts, tv, "parallel", trot, tpos, aspheric0=ta0,
+                           aspheric_stage={0: ta1}, surface0=(g0.curvature, g0.conic),
+                           offsets0=g0.offsets.cpu().numpy())
+    (timg * torch.as_tensor(W)).sum().backward()
+    z = lambda t: (t.grad if t.grad is not None else torch.zeros_like(t)).numpy()  # noqa: E731
+    gt = {"rot": z(trot), "pos": z(tpos), "asph": z(ta0), "asph2": z(ta1), "src": z(ts),
+          "val": z(tv)}
+    keys = ["rot", "pos", "asph", "asph2", "src", "val"]
+    _compare(f"stage2-{'splat' if splat else 'hard'}", gk, gt, keys, tol=3e-3)
