@@ -7,7 +7,9 @@ Drop-in surface of the reference (iactrace/__init__.py:1-19): ``Telescope``,
 batched / differentiable extensions.  Every render runs in the HIP kernels of
 libiactrace_amd.so; there is no other backend.
 """
+from . import filtering
 from .autotune import autotune
+from .filtering import make_filter, partition, show_structure, trainable_params
 from .batched import AlignmentFit, pack_sources, render_batched
 from .geometry import (AsphericSurface, DiskAperture, PolygonAperture, euler_to_matrix,
                        look_at_euler, look_at_rotation)
@@ -34,5 +36,6 @@ __all__ = [
     "CylinderGroup", "Box", "BoxGroup", "Sphere", "SphereGroup", "OrientedBox",
     "OrientedBoxGroup", "Triangle", "TriangleGroup", "group_obstructions", "euler_to_matrix",
     "look_at_euler", "look_at_rotation", "split_key", "render_batched", "pack_sources",
-    "AlignmentFit", "autotune",
+    "AlignmentFit", "autotune", "make_filter", "partition", "trainable_params",
+    "show_structure", "filtering",
 ]
