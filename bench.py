@@ -104,6 +104,28 @@ def c3_fwd_bwd(ia, configs, steps, warmup):
             "bwd_grad": "d(sum(W*image))/d(facet tilts)"}
 
 
+def c3t_culling(ia, configs, steps, warmup):
+    """C3 with a CT5-sized support truss (271 obstructions): forward rays/s
+    with obstruction culling (the default from 16 obstructions) and without."""
+    import torch
+    dev = torch.device("cuda:0")
+    wl = configs.workload("C3T")
+    tel = wl.telescope(dev, key=0)
+    src = torch.as_tensor(wl.sources, dtype=torch.float64, device=dev)
+    vals = torch.as_tensor(wl.values, dtype=torch.float32, device=dev)
+    out = {"workload": f"C3T: C3's dish and camera, {wl.n_facets} facets x {wl.n_samples} "
+                       f"samples x {len(wl.sources)} point sources, "
+                       f"{len(wl.obstructions)} obstructions (masts, bracing, camera mount)",
+           "rays_per_step": wl.n_rays, "steps": steps}
+    for name, cull in (("culled", 1), ("unculled", -1)):
+        L = ia.LaunchConfig(culling=cull)
+        w, _ = time_steps(lambda: ia.render(tel, src, vals, wl.source_type, launch=L), steps,
+                          warmup)
+        out[f"{name}_rays_per_s"] = wl.n_rays * steps / w
+        out[f"{name}_ms"] = 1e3 * w / steps
+    return out
+
+
 def cpu_baseline(wl, tel, target_s=15.0):
     """Time the C/OpenMP port on a source subsample of the same workload."""
     import numpy as np
@@ -272,6 +294,7 @@ def main():
         out["copy_calibration_bytes"] = {"fetch": a.numel() * 4, "write": b.numel() * 4}
     if not args.no_extras and args.config == "C2":
         out["extras"] = {"C3_fwd_bwd": c3_fwd_bwd(ia, configs, 3, 1),
+                         "C3T_culling": c3t_culling(ia, configs, 2, 1),
                          "C5_batched": c5_batched(ia, configs, args.c5_instances)}
     if not args.no_cpu:
         cb, cimg, n = cpu_baseline(wl, tel)

@@ -86,7 +86,8 @@ class IactLaunch(Structure):
     _fields_ = [("block_threads", c_int32), ("source_tile", c_int32),
                 ("blocks_per_cu", c_int32), ("deterministic", c_int32),
                 ("accumulate", c_int32), ("chunk_block", c_int32), ("block_order", c_int32),
-                ("layout", c_int32), ("sample_group", c_int32)]
+                ("layout", c_int32), ("sample_group", c_int32),
+                ("culling", c_int32)]
 
 
 class IactGrads(Structure):

@@ -24,7 +24,7 @@ from .telescope import Telescope
 
 __all__ = ["TelescopeConfigBuilder", "hex_lattice", "hex_facet_vertices", "davies_cotton_dish",
            "parabolic_dish", "hex_camera", "cassegrain", "point_source_field", "Workload",
-           "workload", "BatchWorkload", "batch_workload", "parallel_grid"]
+           "workload", "BatchWorkload", "batch_workload", "parallel_grid", "support_truss"]
 
 
 class TelescopeConfigBuilder:
@@ -278,6 +278,53 @@ def _c3_obstructions(focal=36.0, n_facets=875, pitch=0.91):
     return cyl + [box]
 
 
+def support_truss(focal=36.0, n_facets=875, pitch=0.91, seed=7):
+    """A camera support structure of CT5's size and kind (270 cylinders and
+    the camera body): 4 masts from the dish rim to the camera, each in 16
+    pieces; cross bracing between neighbouring masts at 13 levels (a
+    horizontal and two diagonals per side and level, 156 members); 44 short
+    camera-mount struts and 6 camera supports around the camera."""
+    rng = np.random.default_rng(seed)
+    rim = np.max(np.hypot(*hex_lattice(n_facets, pitch).T))
+    z_cam = 36.25
+    obs = [Box([-1.5, -1.45, z_cam], [1.5, 1.45, z_cam + 1.1])]
+    base = []
+    top = []
+    for k in range(4):
+        a = np.deg2rad(45.0 + 90.0 * k)
+        base.append(np.array([rim * np.cos(a), rim * np.sin(a), rim * rim / (4 * focal)]))
+        top.append(np.array([1.6 * np.cos(a), 1.6 * np.sin(a), z_cam - 0.5]))
+    # masts in pieces
+    for k in range(4):
+        pts = [base[k] + (top[k] - base[k]) * t for t in np.linspace(0, 1, 17)]
+        for i in range(16):
+            obs.append(Cylinder(pts[i], pts[i + 1], 0.162 - 0.005 * i))
+    # bracing between neighbouring masts
+    for lev in range(13):
+        t0, t1 = (lev + 1) / 15.0, (lev + 2) / 15.0
+        for k in range(4):
+            k2 = (k + 1) % 4
+            a0 = base[k] + (top[k] - base[k]) * t0
+            b0 = base[k2] + (top[k2] - base[k2]) * t0
+            a1 = base[k] + (top[k] - base[k]) * t1
+            b1 = base[k2] + (top[k2] - base[k2]) * t1
+            r = 0.03 + 0.03 * rng.random()
+            obs += [Cylinder(a0, b0, r), Cylinder(a0, b1, r), Cylinder(b0, a1, r)]
+    # camera mount: a ring of short struts below the camera, spokes to it
+    ring = [np.array([2.2 * np.cos(a), 2.2 * np.sin(a), z_cam - 0.8])
+            for a in np.linspace(0, 2 * np.pi, 23)[:-1]]
+    for i in range(22):
+        obs.append(Cylinder(ring[i], ring[(i + 1) % 22], 0.05))
+        corner = np.array([np.sign(ring[i][0]) * 1.4, np.sign(ring[i][1]) * 1.4, z_cam])
+        obs.append(Cylinder(ring[i], corner, 0.04))
+    # camera supports
+    for k in range(6):
+        a = 2 * np.pi * k / 6
+        obs.append(Cylinder([1.0 * np.cos(a), 1.0 * np.sin(a), z_cam - 1.5],
+                            [1.0 * np.cos(a), 1.0 * np.sin(a), z_cam], 0.08))
+    return obs
+
+
 def workload(name, n_sources=None, n_samples=None):
     """The benchmark configurations C1..C4 (C5 is C3 batched; see
     iactrace_amd.batched).  Sizes may be reduced for tests."""
@@ -299,6 +346,12 @@ def workload(name, n_sources=None, n_samples=None):
         sensor = HexagonalSensor([0, 0, 36.066], [0, 0, 0], hex_camera(2048))
         src, vals = point_source_field(n_sources or 16384, 2.5, seed=4)
         return Workload(name, parabolic_dish(), _c3_obstructions(), sensor, n_samples or 1024,
+                        src, vals, "point", tilt_sigma_deg=mrad_deg, tilt_seed=3)
+    if name == "C3T":
+        # C3 with a CT5-sized support structure (271 obstructions)
+        sensor = HexagonalSensor([0, 0, 36.066], [0, 0, 0], hex_camera(2048))
+        src, vals = point_source_field(n_sources or 16384, 2.5, seed=4)
+        return Workload(name, parabolic_dish(), support_truss(), sensor, n_samples or 1024,
                         src, vals, "point", tilt_sigma_deg=mrad_deg, tilt_seed=3)
     if name == "C4":
         mirrors, sensor = cassegrain()

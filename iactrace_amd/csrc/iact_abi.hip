@@ -323,6 +323,18 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
                      "the source-major layout covers primary-only, non-deterministic renders");
   if (mode == IACT_MODE_IMAGE && !A.det && sc.n_stages == 0 && (lay == 2 || (lay == 0 && ns >= 64)))
     A.layout = 2;
+  // obstruction culling
+  const int cm = L ? L->culling : 0;
+  if (cm < -1 || cm > 1) return set_error(IACT_ERR_INVALID, "culling mode %d", cm);
+  const int nob = n_obstructions(sc.obstructions);
+  const bool fits = (long long)A.nf * A.M < (1ll << 32);
+  if (cm == 1 && !fits)
+    return set_error(IACT_ERR_UNSUPPORTED, "culling needs facets x samples < 2^32");
+  A.cull = nullptr;
+  A.cull_words = (nob > 0 && fits && (cm == 1 || (cm == 0 && nob >= kAutoCullObstructions)))
+                     ? (nob + 31) / 32
+                     : 0;
+  A.d_m = make_fastdiv((uint32_t)(A.M > 0 ? A.M : 1));
   if (bpc == 0) bpc = default_blocks_per_cu(A.det);
   const long long cap = (long long)dev.n_cu * bpc;
   long long grid;
@@ -368,15 +380,39 @@ int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShap
   return work_order(L, (long long)A.nf * A.cpf, shp.grid, wpb, A);
 }
 
-size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
-                           const TraceArgs& A) {
+static size_t base_workspace_bytes(const iact_scene& sc, int mode, const LaunchShape& shp,
+                                   const TraceArgs& A) {
   if (mode == IACT_MODE_RESPONSE)
     return (size_t)7 * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
   if (mode == IACT_MODE_IMAGE && A.layout == 2)
     return (size_t)IACT_REC_WORDS * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
   if (mode == IACT_MODE_IMAGE && A.det) return (size_t)shp.grid * sc.sensor.n_pixels * sizeof(float);
-  (void)ns;
   return 0;
+}
+
+// Culling tables follow the mode's own workspace: 8 floats of facet cone and
+// cull_words mask words per facet.
+static size_t cull_offset(size_t base) { return (base + 255) & ~(size_t)255; }
+
+size_t cull_bytes(const TraceArgs& A) {
+  return A.cull_words > 0 ? (size_t)A.nf * (8 + A.cull_words) * sizeof(float) : 0;
+}
+
+size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
+                           const TraceArgs& A) {
+  (void)ns;
+  const size_t base = base_workspace_bytes(sc, mode, shp, A);
+  return A.cull_words > 0 ? cull_offset(base) + cull_bytes(A) : base;
+}
+
+int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st) {
+  if (A.cull_words == 0 || A.nf == 0 || A.ns == 0) return IACT_OK;
+  float* cones = (float*)((char*)workspace + cull_offset(base));
+  uint32_t* masks = (uint32_t*)(cones + 8 * (size_t)A.nf);
+  const hipError_t e = launch_cull(A, source_type, cones, masks, st);
+  if (e != hipSuccess) return hip_error(e, "k_facet_cones/k_cull_masks");
+  A.cull = masks;
+  return IACT_OK;
 }
 
 }  // namespace iact
@@ -536,6 +572,10 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
   A.dbg_pix = debug_pixel;
   hipError_t e = hipSuccess;
   const bool empty = (sc.facets.n_facets == 0 || n_sources == 0);
+  if (!empty) {
+    rc = prepare_cull(A, workspace, base_workspace_bytes(sc, mode, shp, A), source_type, st);
+    if (rc) return rc;
+  }
   if (mode == IACT_MODE_IMAGE) {
     if (!(A.det && A.lds_image && !empty)) {
       e = launch_fill(out, P, 0.f, st);

@@ -19,6 +19,12 @@ int default_blocks_per_cu(int det);
 // size; updates the work order that depends on the grid.
 int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShape& shp,
            TraceArgs& A);
+// Obstruction culling: automatic from this many obstructions; tables of
+// cull_bytes() follow a mode's own workspace of `base` bytes, filled by
+// prepare_cull (which also points A.cull at the masks).
+constexpr int kAutoCullObstructions = 16;
+size_t cull_bytes(const TraceArgs& A);
+int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st);
 int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* where);
 int validate_scene(const iact_scene* sc);

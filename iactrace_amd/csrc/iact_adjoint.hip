@@ -304,7 +304,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     const float wgt = A.weights[gi];
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, scale, gi, pw, nw);
-    const Sample smp = make_sample(A.S, pw, nw, wgt);
+    Sample smp = make_sample(A.S, pw, nw, wgt);
+    smp.f = f;
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
     const int s0 = w.s0, s1 = w.s1;
@@ -516,14 +517,22 @@ using namespace iact;
 
 extern "C" {
 
-size_t iact_bwd_workspace_size(const iact_scene*, int32_t, const iact_launch*) { return 0; }
+size_t iact_bwd_workspace_size(const iact_scene* scene, int32_t n_sources,
+                               const iact_launch* launch) {
+  if (validate_scene(scene) || n_sources < 0) return 0;
+  DevInfo dev;
+  if (device_info(dev)) return 0;
+  TraceArgs A;
+  fill_trace_args(A, *scene, nullptr, nullptr, n_sources);
+  LaunchShape shp;
+  if (shape_trace(*scene, n_sources, IACT_MODE_DEBUG, launch, A, shp, dev, -1)) return 0;
+  return cull_bytes(A);
+}
 
 int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const float* values,
                        int32_t n_sources, int32_t source_type, const float* grad_image,
                        const iact_launch* launch, void* workspace, size_t workspace_bytes,
                        const iact_grads* grads, void* stream) {
-  (void)workspace;
-  (void)workspace_bytes;
   int rc = validate_scene(scene);
   if (rc) return rc;
   if (!grads) return set_error(IACT_ERR_INVALID, "null gradient descriptor");
@@ -565,6 +574,10 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   LaunchShape shp;
   rc = shape_trace(sc, n_sources, IACT_MODE_DEBUG, launch, B.T, shp, dev, -1);
   if (rc) return rc;
+  const size_t need = cull_bytes(B.T);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                     workspace_bytes);
   const size_t lds = (size_t)sc.sensor.n_pixels * sizeof(float) + shp.lds;
   if (lds > 65536)
     return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS", lds);
@@ -578,6 +591,8 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   B.surface_index = sc.facets.surface_index;
   B.offsets = sc.facets.offsets;
   rc = regrid(B.T.n_items, launch, dev, shp, B.T);
+  if (rc) return rc;
+  rc = prepare_cull(B.T, workspace, 0, source_type, st);
   if (rc) return rc;
   e = launch_adjoint(B, shp, source_type, st);
   return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_adj");

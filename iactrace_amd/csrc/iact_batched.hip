@@ -151,6 +151,7 @@ static int prepare_batch(const iact_scene* scene, const iact_batch* batch, int s
   iact_launch L = launch ? *launch : iact_launch{};
   L.accumulate = 0;
   L.layout = 1;
+  L.culling = -1;  // the facet cones would depend on each instance's sources
   rc = shape_trace(*scene, max_ns, IACT_MODE_IMAGE, &L, A, shp, dev, -1);
   if (rc) return rc;
   const int wpb = shp.threads / 64;

@@ -395,6 +395,43 @@ IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool activ
   return (blocked && active) ? 0.f : 1.f;
 }
 
+// shadow_factor over a candidate list: bit j of mask word w selects
+// obstruction 32 w + j in the order cylinders, boxes, spheres, oriented
+// boxes, triangles.  The list is a superset of the obstructions the rays can
+// hit, so the result is shadow_factor's.
+IACT_DEV float shadow_factor_culled(const iact_obstructions& ob, cptr<uint32_t> mask, int words,
+                                    f3 o, f3 d, bool active) {
+  const float big = 1e10f;
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
+  const int c3 = c2 + ob.n_oriented_boxes;
+  bool blocked = !active;
+  for (int w = 0; w < words; ++w) {
+    uint32_t bits = mask[w];
+    while (bits != 0u) {
+      if (__all(blocked)) return (blocked && active) ? 0.f : 1.f;
+      const int j = 32 * w + __builtin_ctz(bits);
+      bits &= bits - 1u;
+      float t;
+      if (j < c0)
+        t = hit_cylinder(o, d, as_const(ob.cylinders) + 8 * j);
+      else if (j < c1)
+        t = hit_box(o, d, as_const(ob.boxes) + 6 * (j - c0));
+      else if (j < c2)
+        t = hit_sphere(o, d, as_const(ob.spheres) + 4 * (j - c1));
+      else if (j < c3)
+        t = hit_oriented_box(o, d, as_const(ob.oriented_boxes) + 15 * (j - c2));
+      else
+        t = hit_triangle(o, d, as_const(ob.triangles) + 9 * (j - c3));
+      blocked |= t < big;
+    }
+  }
+  return (blocked && active) ? 0.f : 1.f;
+}
+
+__host__ __device__ inline int n_obstructions(const iact_obstructions& ob) {
+  return ob.n_cylinders + ob.n_boxes + ob.n_spheres + ob.n_oriented_boxes + ob.n_triangles;
+}
+
 __host__ __device__ inline bool has_obstructions(const iact_obstructions& ob) {
   return (ob.n_cylinders | ob.n_boxes | ob.n_spheres | ob.n_oriented_boxes | ob.n_triangles) != 0;
 }

@@ -128,6 +128,173 @@ __global__ void k_facet_world(iact_facets F, DevSensor S, float* out) {
   }
 }
 
+// ------------------------------------------------------------------ culling
+// Per facet, the bundle of primary-leg shadow rays: every ray starts inside
+// the ball of radius rho around the facet position (rho = the largest sample
+// distance in the facet frame) and points inside the cone of half angle
+// theta around axis a, the directions towards the sources (for point
+// sources widened by asin(rho / distance)).  cones[8 f ..]: position,
+// rho, a, tan(theta); tan < 0 disables culling for the facet.
+__global__ __launch_bounds__(256) void k_facet_cones(TraceArgs A, int source_type, float* cones) {
+  __shared__ float red[4][256];
+  const int t = threadIdx.x;
+  for (int f = blockIdx.x; f < A.nf; f += gridDim.x) {
+    const f3 c = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
+    float rho = 0.f;
+    for (int m = t; m < A.M; m += blockDim.x) {
+      const size_t gi = (size_t)f * A.M + m;
+      const f3 pl = mk3(A.points[3 * gi], A.points[3 * gi + 1], A.points[3 * gi + 2]);
+      rho = fmaxf(rho, sqrtf(dot(pl, pl)));
+    }
+    // axis: mean of the unit directions towards the sources
+    f3 sum = mk3(0.f, 0.f, 0.f);
+    for (int s = t; s < A.ns; s += blockDim.x) {
+      f3 u;
+      if (source_type == IACT_SOURCE_POINT) {
+        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
+        const double uz = A.sources[3 * s + 2] - (double)c.z;
+        const double il = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
+        u = mk3((float)(ux * il), (float)(uy * il), (float)(uz * il));
+      } else {
+        u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
+                -(float)A.sources[3 * s + 2]);
+        u = frsq(dot(u, u)) * u;
+      }
+      sum = sum + u;
+    }
+    red[0][t] = sum.x;
+    red[1][t] = sum.y;
+    red[2][t] = sum.z;
+    red[3][t] = rho;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+      if (t < h) {
+        red[0][t] += red[0][t + h];
+        red[1][t] += red[1][t + h];
+        red[2][t] += red[2][t + h];
+        red[3][t] = fmaxf(red[3][t], red[3][t + h]);
+      }
+      __syncthreads();
+    }
+    const f3 sa = mk3(red[0][0], red[1][0], red[2][0]);
+    const float rho_f = red[3][0];
+    const float la = sqrtf(dot(sa, sa));
+    const f3 a = la > 0.f ? (1.f / la) * sa : mk3(0.f, 0.f, 1.f);
+    __syncthreads();
+    // half angle, including the parallax of point sources
+    float ang = 0.f;
+    for (int s = t; s < A.ns; s += blockDim.x) {
+      float par = 0.f;
+      f3 u;
+      if (source_type == IACT_SOURCE_POINT) {
+        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
+        const double uz = A.sources[3 * s + 2] - (double)c.z;
+        const double l = sqrt(ux * ux + uy * uy + uz * uz);
+        u = mk3((float)(ux / l), (float)(uy / l), (float)(uz / l));
+        par = l > (double)rho_f ? asinf((float)((double)rho_f / l)) : 3.1415927f;
+      } else {
+        u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
+                -(float)A.sources[3 * s + 2]);
+        u = frsq(dot(u, u)) * u;
+      }
+      ang = fmaxf(ang, acosf(fminf(fmaxf(dot(u, a), -1.f), 1.f)) + par);
+    }
+    red[0][t] = ang;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+      if (t < h) red[0][t] = fmaxf(red[0][t], red[0][t + h]);
+      __syncthreads();
+    }
+    if (t == 0) {
+      // 2e-4 rad of slack for the fp32 directions of the trace
+      const float th = red[0][0] + 2e-4f;
+      float* o = cones + 8 * (size_t)f;
+      o[0] = c.x;
+      o[1] = c.y;
+      o[2] = c.z;
+      o[3] = rho_f;
+      o[4] = a.x;
+      o[5] = a.y;
+      o[6] = a.z;
+      o[7] = th < 1.4f ? tanf(th) : -1.f;  // beyond ~80 degrees: no culling
+    }
+    __syncthreads();
+  }
+}
+
+// Can a ray of the bundle (centre c, radius rho, axis a, tan theta) pass
+// within R of b?  Along the axis the ray point lies in [-rho + t cos, rho + t],
+// across it within rho + t sin of the axis, so it is necessary that
+// s + rho + R >= 0 and e <= rho + R + tan(theta) (s + rho + R), with s, e the
+// axial and radial coordinates of b.  Radii get 0.1% + 1 mm of slack.
+IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
+  const f3 w = b - c;
+  const float s = dot(w, a);
+  const f3 perp = fma3(-s, a, w);
+  const float e = sqrtf(dot(perp, perp));
+  const float rr = fmaf(R, 1.001f, 1e-3f) + rho;
+  // written so that NaN geometry keeps the candidate
+  return !(s + rr < 0.f) && !(e > fmaf(tn, s + rr, rr));
+}
+
+// Candidate bits: one thread per (facet, mask word).  Long cylinders are
+// tested as a chain of bounding spheres of pieces at most 1 m long (at most
+// 64 pieces).
+__global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks) {
+  const iact_obstructions& ob = A.ob;
+  const int words = A.cull_words;
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
+  const int c3 = c2 + ob.n_oriented_boxes, c4 = c3 + ob.n_triangles;
+  const long long total = (long long)A.nf * words;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(id / words), w = (int)(id - (long long)f * words);
+    const float* cn = cones + 8 * (size_t)f;
+    const f3 c = mk3(cn[0], cn[1], cn[2]);
+    const float rho = cn[3], tn = cn[7];
+    const f3 a = mk3(cn[4], cn[5], cn[6]);
+    uint32_t bits = 0u;
+    for (int j = 32 * w; j < min(32 * w + 32, c4); ++j) {
+      bool hit;
+      if (tn < 0.f) {
+        hit = true;
+      } else if (j < c0) {
+        const float* q = ob.cylinders + 8 * j;
+        const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
+        const float h = q[6], r = q[7];
+        const int n = (int)fminf(fmaxf(ceilf(h), 1.f), 64.f);
+        const float hp = h / (float)n;
+        const float R = sqrtf(0.25f * hp * hp + r * r);
+        hit = false;
+        for (int i = 0; i < n && !hit; ++i)
+          hit = bundle_reaches(c, rho, a, tn, fma3(hp * ((float)i + 0.5f), ax, p1), R);
+      } else if (j < c1) {
+        const float* q = ob.boxes + 6 * (j - c0);
+        const f3 lo = mk3(q[0], q[1], q[2]), hi = mk3(q[3], q[4], q[5]);
+        const f3 dd = hi - lo;
+        hit = bundle_reaches(c, rho, a, tn, 0.5f * (lo + hi), 0.5f * sqrtf(dot(dd, dd)));
+      } else if (j < c2) {
+        const float* q = ob.spheres + 4 * (j - c1);
+        hit = bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), q[3]);
+      } else if (j < c3) {
+        const float* q = ob.oriented_boxes + 15 * (j - c2);
+        const f3 he = mk3(q[3], q[4], q[5]);
+        hit = bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), sqrtf(dot(he, he)));
+      } else {
+        const float* q = ob.triangles + 9 * (j - c3);
+        const f3 v0 = mk3(q[0], q[1], q[2]), v1 = mk3(q[3], q[4], q[5]);
+        const f3 v2 = mk3(q[6], q[7], q[8]);
+        const f3 ctr = (1.f / 3.f) * (v0 + v1 + v2);
+        const f3 e0 = v0 - ctr, e1 = v1 - ctr, e2 = v2 - ctr;
+        const float R = sqrtf(fmaxf(fmaxf(dot(e0, e0), dot(e1, e1)), dot(e2, e2)));
+        hit = bundle_reaches(c, rho, a, tn, ctr, R);
+      }
+      if (hit) bits |= 1u << (j - 32 * w);
+    }
+    masks[id] = bits;
+  }
+}
+
 // ------------------------------------------------------------------ binning
 // Sensor.accumulate on given sensor-plane coordinates.
 template <int KIND, int SPLAT>
@@ -232,7 +399,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
-    const Sample smp = make_sample(A.S, pw, nw, A.weights[gi]);
+    Sample smp = make_sample(A.S, pw, nw, A.weights[gi]);
+    smp.f = f;
     const int s0 = w.s0, s1 = w.s1;
     float* const gout = A.out + (size_t)w.inst * P;
     const cptr<double> src = as_const(A.sources + 3 * w.sbase);
@@ -369,8 +537,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const float4* recs = (const float4*)A.recs;
     int run_pix = -1;
     float run_v = 0.f;
-    auto trace_one = [&](const RecRaw& rec) {
-      const Sample smp = unpack_record(rec);
+    auto trace_one = [&](const RecRaw& rec, long long j) {
+      Sample smp = unpack_record(rec);
+      smp.f = A.cull ? (int)fdiv((uint32_t)j, A.d_m) : 0;
       float x, y, v;
       const bool ok =
           trace_ray<SRC, STAGES, AXIS, OBST>(A, smp, active, sx, sy, sz, value, x, y, v);
@@ -396,10 +565,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     RecRaw ra = fetch_record(recs + 4 * j0), rb;
     for (long long j = j0; j < j1; j += 2) {
       rb = fetch_record(recs + 4 * (j + 1 < j1 ? j + 1 : j));
-      trace_one(ra);
+      trace_one(ra, j);
       if (j + 1 >= j1) break;
       ra = fetch_record(recs + 4 * (j + 2 < j1 ? j + 2 : j + 1));
-      trace_one(rb);
+      trace_one(rb, j + 1);
     }
     if (!SPLAT && run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
   }
@@ -438,13 +607,14 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
     const double sx = A.sources[3 * s], sy = A.sources[3 * s + 1], sz = A.sources[3 * s + 2];
     const float val = A.values[s];
     for (long long c = wave; c < nchunks; c += wpb) {
-      const int f = (int)(c / A.cpf);
-      const int m = (int)(c % A.cpf) * 64 + lane;
+      const int f = (int)fdiv((uint32_t)c, A.d_cpf);
+      const int m = ((int)c - f * A.cpf) * 64 + lane;
       const bool active = m < A.M;
       const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
       const f3 p = mk3(A.world[gi], A.world[total + gi], A.world[2 * total + gi]);
       const f3 n = mk3(A.world[3 * total + gi], A.world[4 * total + gi], A.world[5 * total + gi]);
-      const Sample smp = make_sample(A.S, p, n, A.world[6 * total + gi]);
+      Sample smp = make_sample(A.S, p, n, A.world[6 * total + gi]);
+      smp.f = f;
       float x, y, v;
       const bool ok = trace_ray<SRC, true, false>(A, smp, active, sx, sy, sz, val, x, y, v);
       if (!SPLAT) {
@@ -575,6 +745,18 @@ hipError_t launch_trace(const TraceArgs& A, const LaunchShape& L, int source_typ
 hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_type,
                            hipStream_t st) {
   return launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR), A, L, st);
+}
+
+hipError_t launch_cull(const TraceArgs& A, int source_type, float* cones, uint32_t* masks,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_facet_cones, dim3(grid_for(A.nf, 1, 4096)), dim3(256), 0, st, A,
+                     source_type, cones);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long total = (long long)A.nf * A.cull_words;
+  hipLaunchKernelGGL(k_cull_masks, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, A, cones,
+                     masks);
+  return hipGetLastError();
 }
 
 hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st) {

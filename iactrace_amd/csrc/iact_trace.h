@@ -100,7 +100,12 @@ struct TraceArgs {
   int sample_group;              // samples per work item
   uint32_t n_sc;                 // source chunks of 64
   FastDiv d_sc;                  // divide by n_sc
+  FastDiv d_m;                   // divide by M (sample -> facet)
   long long sm_items;            // sample groups * source chunks
+  // per-facet obstruction candidate bit masks for the primary leg's shadow
+  // rays (NULL: every obstruction is tested)
+  const uint32_t* cull;
+  int cull_words;
 };
 
 // World-space sample record of the source-major trace (16 words, one scalar
@@ -208,6 +213,7 @@ struct Sample {
   float nrw;       // -1 / weight (the reflected value is value * cos / w, cos = -d.n)
   double px, py, pz;
   PlaneK K;        // sensor-plane constants of the primary origin
+  int f;           // facet (selects its obstruction candidate list)
 };
 
 IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w) {
@@ -219,6 +225,7 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w) {
   s.py = (double)p.y;
   s.pz = (double)p.z;
   s.K = plane_consts(S, p);
+  s.f = 0;
   return s;
 }
 
@@ -422,7 +429,9 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, double
   f3 r = fma3(-2.f * c, s.n, d);
   v = value * c * s.nrw;
   if (OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))
-    v *= shadow_factor(A.ob, s.p, -d, active);
+    v *= A.cull ? shadow_factor_culled(A.ob, as_const(A.cull) + (size_t)s.f * A.cull_words,
+                                       A.cull_words, s.p, -d, active)
+                : shadow_factor(A.ob, s.p, -d, active);
   PlaneK K = s.K;
   if (STAGES && A.n_stages > 0) {
     f3 o = s.p;

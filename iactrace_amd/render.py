@@ -31,7 +31,8 @@ class LaunchConfig:
     """Launch shape handed to the kernels (see iactrace_amd.autotune)."""
 
     def __init__(self, block_threads=0, source_tile=0, blocks_per_cu=0, deterministic=False,
-                 accumulate=0, chunk_block=0, block_order=0, layout=0, sample_group=0):
+                 accumulate=0, chunk_block=0, block_order=0, layout=0, sample_group=0,
+                 culling=0):
         self.block_threads = int(block_threads)
         self.source_tile = int(source_tile)
         self.blocks_per_cu = int(blocks_per_cu)
@@ -41,6 +42,7 @@ class LaunchConfig:
         self.block_order = int(block_order)
         self.layout = int(layout)
         self.sample_group = int(sample_group)
+        self.culling = int(culling)
 
     def abi(self):
         L = _lib.IactLaunch()
@@ -53,6 +55,7 @@ class LaunchConfig:
         L.block_order = self.block_order
         L.layout = self.layout
         L.sample_group = self.sample_group
+        L.culling = self.culling
         return L
 
     def __repr__(self):
@@ -60,7 +63,8 @@ class LaunchConfig:
                 f"{self.source_tile}, blocks_per_cu={self.blocks_per_cu}, "
                 f"deterministic={self.deterministic}, accumulate={self.accumulate}, "
                 f"chunk_block={self.chunk_block}, block_order={self.block_order}, "
-                f"layout={self.layout}, sample_group={self.sample_group})")
+                f"layout={self.layout}, sample_group={self.sample_group}, "
+                f"culling={self.culling})")
 
 
 def _stages(groups):

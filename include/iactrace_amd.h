@@ -161,6 +161,13 @@ typedef struct iact_launch {
                               workspace (one LDS add per ray; best for many
                               sources spread over the sensor)               */
   int32_t sample_group;    /* layout 2: samples per work item (0 = 64)     */
+  int32_t culling;         /* obstruction culling: a prepass bounds, per
+                              facet, the bundle of source-facing rays and
+                              keeps the obstructions it can reach (a bit
+                              mask per facet in the workspace); shadows of
+                              the other obstructions are not tested.  0 =
+                              automatic (on from 16 obstructions), 1 = on,
+                              -1 = off.  Results are identical either way. */
 } iact_launch;
 
 /* Gradient outputs (fp64, overwritten).  NULL = not requested.               */
