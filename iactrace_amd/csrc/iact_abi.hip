@@ -79,6 +79,8 @@ int validate_sensor(const iact_sensor& S) {
   if (S.kind != IACT_SENSOR_SQUARE && S.kind != IACT_SENSOR_HEX)
     return set_error(IACT_ERR_INVALID, "unknown sensor kind %d", S.kind);
   if (S.n_pixels <= 0) return set_error(IACT_ERR_INVALID, "sensor has no pixels");
+  if (S.n_pixels >= (1 << 24))
+    return set_error(IACT_ERR_UNSUPPORTED, "sensor with %d pixels (max 2^24 - 1)", S.n_pixels);
   if (!finite3(S.position)) return set_error(IACT_ERR_INVALID, "non-finite sensor position");
   for (int i = 0; i < 9; ++i)
     if (!isfinite(S.rotation[i])) return set_error(IACT_ERR_INVALID, "non-finite sensor frame");
@@ -165,6 +167,7 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
   S.inv_dy = s.dy > 0.f ? (float)(1.0 / (double)s.dy) : 0.f;
   S.nx0 = s.dx > 0.f ? (float)(-(double)s.x0 / (double)s.dx) : 0.f;
   S.ny0 = s.dy > 0.f ? (float)(-(double)s.y0 / (double)s.dy) : 0.f;
+  S.v_scale = S.inv_dx > 0.f ? S.inv_dy / S.inv_dx : 1.f;
   S.wf = (float)s.width;
   S.hf = (float)s.height;
   S.edge_width = s.edge_width;
@@ -318,10 +321,14 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
   const int lay = L ? L->layout : 0;
   if (lay < 0 || lay > 2) return set_error(IACT_ERR_INVALID, "layout %d", lay);
   A.layout = 1;
-  if (mode == IACT_MODE_IMAGE && lay == 2 && (A.det || sc.n_stages > 0))
+  // the source-major trace indexes samples in 31 bits
+  const bool sm_fits = (long long)A.nf * A.M < (1ll << 31);
+  if (mode == IACT_MODE_IMAGE && lay == 2 && (A.det || sc.n_stages > 0 || !sm_fits))
     return set_error(IACT_ERR_UNSUPPORTED,
-                     "the source-major layout covers primary-only, non-deterministic renders");
-  if (mode == IACT_MODE_IMAGE && !A.det && sc.n_stages == 0 && (lay == 2 || (lay == 0 && ns >= 64)))
+                     "the source-major layout covers primary-only, non-deterministic renders "
+                     "of fewer than 2^31 samples");
+  if (mode == IACT_MODE_IMAGE && !A.det && sc.n_stages == 0 && sm_fits &&
+      (lay == 2 || (lay == 0 && ns >= 64)))
     A.layout = 2;
   // obstruction culling
   const int cm = L ? L->culling : 0;
@@ -345,6 +352,7 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
     const long long n_sc = ns > 0 ? (ns + 63) / 64 : 1;
     // items are decoded in 32 bits
     while (((total + sg - 1) / sg) * n_sc >= (1ll << 31)) sg *= 2;
+    if (sg > total) sg = total > 0 ? total : 1;
     A.sample_group = (int)sg;
     A.n_sc = (uint32_t)n_sc;
     A.d_sc = make_fastdiv((uint32_t)n_sc);
@@ -385,7 +393,9 @@ static size_t base_workspace_bytes(const iact_scene& sc, int mode, const LaunchS
   if (mode == IACT_MODE_RESPONSE)
     return (size_t)7 * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
   if (mode == IACT_MODE_IMAGE && A.layout == 2)
-    return (size_t)IACT_REC_WORDS * sc.facets.n_facets * sc.facets.n_samples * sizeof(float);
+    // two records of padding: the trace prefetches up to two records ahead
+    return (size_t)IACT_REC_WORDS * ((size_t)sc.facets.n_facets * sc.facets.n_samples + 2) *
+           sizeof(float);
   if (mode == IACT_MODE_IMAGE && A.det) return (size_t)shp.grid * sc.sensor.n_pixels * sizeof(float);
   return 0;
 }

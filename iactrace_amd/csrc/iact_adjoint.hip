@@ -304,50 +304,52 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     const float wgt = A.weights[gi];
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, scale, gi, pw, nw);
-    Sample smp = make_sample(A.S, pw, nw, wgt);
+    Sample smp = make_sample<KIND>(A.S, pw, nw, wgt, pos);
     smp.f = f;
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
     const int s0 = w.s0, s1 = w.s1;
-    const cptr<double> src = as_const(A.sources + 3 * w.sbase);
+    const double* src = A.sources + 3 * w.sbase;
     const cptr<float> vals = as_const(A.values + w.sbase);
+    f3 e_lane = mk3(0.f, 0.f, 0.f);  // source offsets, 64 at a time (see k_trace)
     for (int s = s0; s < s1; ++s) {
-      const double sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+      const int k = (s - s0) & 63;
+      if (k == 0) {
+        const int sl = min(s + lane, s1 - 1);
+        e_lane = source_offset<SRC>(src[3 * sl], src[3 * sl + 1], src[3 * sl + 2], pos);
+      }
+      const f3 e = lane_value(e_lane, k);
       const float value = vals[s];
       // ---- forward recompute (the operations of trace_ray)
-      f3 d, u;
-      float inv = 1.f;
-      if (SRC == IACT_SOURCE_POINT) {
-        u = mk3((float)(smp.px - sx), (float)(smp.py - sy), (float)(smp.pz - sz));
-        inv = frsq(fmaf(u.x, u.x, fmaf(u.y, u.y, u.z * u.z)));
-        d = mk3(u.x * inv, u.y * inv, u.z * inv);
-      } else {
-        d = mk3((float)sx, (float)sy, (float)sz);
-      }
+      float inv;
+      const f3 d = primary_direction<SRC>(smp, e, inv);
       const float c = dot(d, smp.n);
       const f3 r = fma3(-2.f * c, smp.n, d);
-      const float shadow =
-          has_obstructions(A.ob) ? shadow_factor(A.ob, smp.p, -d, active) : 1.f;
+      const float shadow = primary_shadow<2>(A, smp, d, active);
       const float vunit = active ? c * smp.nrw * shadow : 0.f;  // ray value / source value
       // stages >= 1 (render.py:44-79), recorded for their adjoint
       f3 of = smp.p, rf = r;
       float vf = value * vunit;
       StageRec rec[IACT_ADJ_MAX_STAGES];
       PlaneK K = smp.K;
+      GridK Kg = smp.G;
       if (STAGES) {
 #pragma unroll
         for (int k = 0; k < IACT_ADJ_MAX_STAGES; ++k)
           if (k < A.n_stages) stage_forward<true>(A, A.stages[k], of, rf, vf, active, rec[k]);
         K = plane_consts(A.S, of);
+        Kg = grid_consts<KIND>(A.S, K);
       }
-      float x, y;
-      const bool ok = plane_xy<false>(A.S, K, rf, x, y) && active;
-      // ---- image adjoint -> ray adjoints
+      // ---- image adjoint -> ray adjoints (pixels picked as the forward does)
       float gv, gx = 0.f, gy = 0.f;
       if (!SPLAT) {
-        const int pix = ok ? hard_pixel<KIND>(A.S, tab, x, y) : -1;
+        float gu_, gv_;
+        const bool ok = plane_grid<KIND, false>(A.S, Kg, rf, gu_, gv_) && active;
+        const int pix = ok ? grid_pixel<KIND>(A.S, tab, gu_, gv_) : -1;
         gv = pix >= 0 ? G[pix] : 0.f;
       } else {
+        float x, y;
+        const bool ok = plane_xy<false>(A.S, K, rf, x, y) && active;
         gv = splat_adjoint<KIND>(A.S, tab, G, x, y, ok, vf, gx, gy);
       }
       // the sensor-plane hit depends on the last origin and direction

@@ -444,6 +444,7 @@ struct DevSensor {
   f3 pos, u1, u2, nrm;
   float ndotp;
   float nx0, ny0;  // -x0/dx, -y0/dy: pixel coordinate = fma(x, 1/dx, -x0/dx)
+  float v_scale;   // (1/dy) / (1/dx): 1 for square pixels
   float aqx, aqy, aq0, arx, ary, ar0;  // sensor (x, y) -> fractional axial (q, r)
   // square
   int width, height;
@@ -490,17 +491,66 @@ IACT_DEV bool plane_xy(const DevSensor& S, const PlaneK& K, f3 r, float& x, floa
   return !(fabsf(nd) < 1e-10f) && t > 0.f;
 }
 
-// Square binning (square.py:66-91): floor, range, edge distance.
-IACT_DEV int square_pixel(const DevSensor& S, float x, float y) {
-  const float xc = fmaf(x, S.inv_dx, S.nx0), yc = fmaf(y, S.inv_dy, S.ny0);
+// Sensor-grid constants of an origin.  The plane hit (x, y) = (kx + t ru,
+// ky + t rv) enters the binning only through an affine map -- pixel units
+// ((x - x0)/dx, (y - y0)/dy) for square sensors, fractional axial (q, r)
+// for hexagonal ones -- so the origin's part folds into (ku, kv) and a ray
+// is left with t times the grid components of its direction.  Square
+// sensors also fold 1/dx into kt (the sign of t is unchanged).
+struct GridK {
+  float kt, ku, kv;
+};
+template <int KIND>
+IACT_DEV GridK grid_consts(const DevSensor& S, const PlaneK& K) {
+  if (KIND == IACT_SENSOR_SQUARE)
+    return GridK{K.kt * S.inv_dx, fmaf(K.kx, S.inv_dx, S.nx0), fmaf(K.ky, S.inv_dy, S.ny0)};
+  return GridK{K.kt, fmaf(K.kx, S.aqx, fmaf(K.ky, S.aqy, S.aq0)),
+               fmaf(K.kx, S.arx, fmaf(K.ky, S.ary, S.ar0))};
+}
+
+// plane_xy in grid coordinates (gu, gv); the same miss conditions.
+template <int KIND, bool AXIS>
+IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, float& gv) {
+  float nd, ru, rv;
+  if (AXIS) {
+    nd = r.z;
+    ru = r.x;
+    rv = r.y;
+  } else {
+    nd = dot(r, S.nrm);
+    ru = dot(r, S.u1);
+    rv = dot(r, S.u2);
+  }
+  const float t = G.kt * frcp(nd);
+  if (KIND == IACT_SENSOR_SQUARE) {
+    gu = fmaf(t, ru, G.ku);
+    gv = fmaf(t, rv * S.v_scale, G.kv);
+  } else {
+    gu = fmaf(t, fmaf(ru, S.aqx, rv * S.aqy), G.ku);
+    gv = fmaf(t, fmaf(ru, S.arx, rv * S.ary), G.kv);
+  }
+  return !(fabsf(nd) < 1e-10f) && t > 0.f;
+}
+
+// Square binning (square.py:66-91) of pixel coordinates (xc, yc): floor,
+// range, edge distance.  The edge test, when the sensor has one, acts on the
+// pixel id inside a launch-uniform branch.
+IACT_DEV int square_pixel_grid(const DevSensor& S, float xc, float yc) {
   const float xf = floorf(xc), yf = floorf(yc);
-  bool ok = (xf >= 0.f) && (xf < S.wf) && (yf >= 0.f) && (yf < S.hf);
+  // bitwise ands: four compares and no branch
+  const bool ok = (xf >= 0.f) & (xf < S.wf) & (yf >= 0.f) & (yf < S.hf);
+  // row-major index in fp32: exact, the sensor having fewer than 2^24 pixels
+  int pix = ok ? (int)fmaf(yf, S.wf, xf) : -1;
   if (S.edge_width > 0.f) {
     const float xr = xc - xf, yr = yc - yf;
     const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
-    ok = ok && !(dist < S.edge_width);
+    pix = dist < S.edge_width ? -1 : pix;
   }
-  return ok ? (int)yf * S.width + (int)xf : -1;
+  return pix;
+}
+
+IACT_DEV int square_pixel(const DevSensor& S, float x, float y) {
+  return square_pixel_grid(S, fmaf(x, S.inv_dx, S.nx0), fmaf(y, S.inv_dy, S.ny0));
 }
 
 // Sensor (x, y) -> fractional axial (q, r): the grid offset, the grid
@@ -537,14 +587,19 @@ IACT_DEV float hex_norm_axial(float dq, float dr) {
   return fmaxf(2.f * a, fmaf(1.5f, b, a));
 }
 
-// Hard hex binning (hexagonal.py:174-194).
-IACT_DEV int hex_pixel(const DevSensor& S, const int32_t* table, float x, float y) {
-  float q, r, qi, ri;
-  hex_axial(S, x, y, q, r);
+// Hard hex binning (hexagonal.py:174-194) of fractional axial (q, r).
+IACT_DEV int hex_pixel_grid(const DevSensor& S, const int32_t* table, float q, float r) {
+  float qi, ri;
   hex_round(q, r, qi, ri);
   const int pid = hex_lookup(S, table, qi, ri);
   const float hd = hex_norm_axial(q - qi, r - ri);
   return (pid >= 0 && !(hd > S.edge_thr)) ? pid : -1;
+}
+
+IACT_DEV int hex_pixel(const DevSensor& S, const int32_t* table, float x, float y) {
+  float q, r;
+  hex_axial(S, x, y, q, r);
+  return hex_pixel_grid(S, table, q, r);
 }
 
 // Add to a block's LDS image: relaxed, workgroup scope (the image is private

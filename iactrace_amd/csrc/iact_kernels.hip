@@ -319,7 +319,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_accumulate(DevSensor S, cons
       wave_accumulate<true>(out, pix, vi, 16);
     } else {
       if (pixel && active) pixel[i] = hard_pixel<KIND>(S, tab, xi, yi);
-      splat_taps<KIND>(S, tab, xi, yi, active,
+      float gu, gv;
+      to_grid<KIND>(S, xi, yi, gu, gv);
+      splat_taps<KIND>(S, tab, gu, gv, active,
                        [&](int pix, float w) { wave_accumulate<true>(out, pix, vi * w, 16); });
     }
   }
@@ -399,24 +401,28 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
-    Sample smp = make_sample(A.S, pw, nw, A.weights[gi]);
+    Sample smp = make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos);
     smp.f = f;
     const int s0 = w.s0, s1 = w.s1;
     float* const gout = A.out + (size_t)w.inst * P;
-    const cptr<double> src = as_const(A.sources + 3 * w.sbase);
+    const double* src = A.sources + 3 * w.sbase;
     const cptr<float> vals = as_const(A.values + w.sbase);
-    double sx = src[3 * s0], sy = src[3 * s0 + 1], sz = src[3 * s0 + 2];
+    // the facet's source offsets, 64 sources at a time (lane k holds source
+    // s + k), read by the wave one by one
+    f3 e_lane = mk3(0.f, 0.f, 0.f);
     float sv = vals[s0];
     for (int s = s0; s < s1; ++s) {
-      // prefetch the next source while this one is traced
-      const int sn = s + 1 < s1 ? s + 1 : s;
-      const double nx = src[3 * sn], ny = src[3 * sn + 1], nz = src[3 * sn + 2];
-      const float nv = vals[sn];
-      float x, y, v;
-      const bool ok = trace_ray<SRC, STAGES, AXIS>(A, smp, active, sx, sy, sz, sv, x, y, v);
-      sx = nx;
-      sy = ny;
-      sz = nz;
+      const int k = (s - s0) & 63;
+      if (k == 0) {
+        const int sl = min(s + lane, s1 - 1);
+        e_lane = source_offset<SRC>(src[3 * sl], src[3 * sl + 1], src[3 * sl + 2], pos);
+      }
+      const f3 e = lane_value(e_lane, k);
+      // prefetch the next value while this source is traced
+      const float nv = vals[s + 1 < s1 ? s + 1 : s];
+      float gu, gv, v, x, y;
+      const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, 2, MODE == IACT_MODE_DEBUG>(
+          A, smp, active, e, sv, gu, gv, v, x, y);
       sv = nv;
       if (MODE == IACT_MODE_DEBUG) {
         if (active) {
@@ -424,10 +430,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
           A.out[idx] = v;
           A.dbg_xy[2 * idx] = ok ? x : 1e10f;
           A.dbg_xy[2 * idx + 1] = ok ? y : 1e10f;
-          if (A.dbg_pix) A.dbg_pix[idx] = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+          if (A.dbg_pix) A.dbg_pix[idx] = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
         }
       } else if (!SPLAT) {
-        const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        const int pix = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
         if (A.lds_image && A.acc == 1) {
           // per-lane run of equal pixels, flushed when the pixel changes
           if (pix != run_pix) {
@@ -435,7 +441,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
             run_pix = pix;
             run_v = 0.f;
           }
-          run_v += pix >= 0 ? v : 0.f;
+          run_v += v;  // a run of pix = -1 is never added
         } else if (A.lds_image && A.acc == 2) {
           if (pix >= 0 && v != 0.f) atomicAdd(img + pix, v);
         } else if (A.lds_image) {
@@ -444,7 +450,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
           wave_accumulate<true>(gout, pix, v, A.max_rounds);
         }
       } else {
-        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float wt) {
+        splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float wt) {
           if (A.lds_image)
             wave_accumulate<false>(img, pix, v * wt, A.max_rounds);
           else
@@ -473,6 +479,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
 }
 
 // World-space records of every (facet, sample) for the source-major trace.
+template <int KIND>
 __global__ void k_sample_records(TraceArgs A, float* recs) {
   const long long total = (long long)A.nf * A.M;
   for (long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x; gi < total;
@@ -482,7 +489,7 @@ __global__ void k_sample_records(TraceArgs A, float* recs) {
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], (size_t)gi, pw, nw);
-    store_record(recs + IACT_REC_WORDS * gi, make_sample(A.S, pw, nw, A.weights[gi]));
+    store_record(recs + IACT_REC_WORDS * gi, make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos));
   }
 }
 
@@ -528,33 +535,48 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const uint32_t g = fdiv((uint32_t)item, A.d_sc);
     const int c = (int)((uint32_t)item - g * A.n_sc);
     const int s = c * 64 + lane;
-    const bool active = s < A.ns;
-    const int sl = active ? s : A.ns - 1;
+    // lanes past the last source trace its rays with value 0 (their runs
+    // are never added), which keeps every lane of the trace active
+    const int sl = s < A.ns ? s : A.ns - 1;
     const double sx = A.sources[3 * sl], sy = A.sources[3 * sl + 1], sz = A.sources[3 * sl + 2];
-    const float value = A.values[sl];
-    const long long j0 = (long long)g * A.sample_group;
-    const long long j1 = min(j0 + A.sample_group, total);
+    const float value = s < A.ns ? A.values[sl] : 0.f;
+    // sample indices fit in 31 bits (checked on the host): scalar compares
+    const int j0 = (int)g * A.sample_group;
+    const int j1 = (int)min((long long)j0 + A.sample_group, total);
     const float4* recs = (const float4*)A.recs;
     int run_pix = -1;
     float run_v = 0.f;
-    auto trace_one = [&](const RecRaw& rec, long long j) {
+    // facet of the current samples and the lane source's offset from it
+    int fend = -1;
+    int fcur = 0;
+    f3 e = mk3(0.f, 0.f, 0.f);
+    auto trace_one = [&](const RecRaw& rec, int j) {
+      if (j >= fend) {
+        fcur = (int)fdiv((uint32_t)j, A.d_m);
+        fend = (fcur + 1) * A.M;
+        const cptr<float> fp = as_const(A.positions) + 3 * fcur;  // scalar loads
+        const f3 pos = mk3(fp[0], fp[1], fp[2]);
+        e = source_offset<SRC>(sx, sy, sz, pos);
+      }
       Sample smp = unpack_record(rec);
-      smp.f = A.cull ? (int)fdiv((uint32_t)j, A.d_m) : 0;
-      float x, y, v;
+      smp.f = fcur;
+      float gu, gv, v, x, y;
       const bool ok =
-          trace_ray<SRC, STAGES, AXIS, OBST>(A, smp, active, sx, sy, sz, value, x, y, v);
+          trace_ray<SRC, KIND, STAGES, AXIS, OBST>(A, smp, true, e, value, gu, gv, v, x, y);
       if (!SPLAT) {
         // run of equal pixels per lane (consecutive samples of a facet put a
-        // source's rays on the same one or two pixels), added when it ends
-        const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        // source's rays on the same one or two pixels), added when it ends;
+        // the pixel is found for every lane and then selected
+        const int pg = grid_pixel<KIND>(A.S, table, gu, gv);
+        const int pix = ok ? pg : -1;
         if (pix != run_pix) {
           if (run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
           run_pix = pix;
           run_v = 0.f;
         }
-        run_v += pix >= 0 ? v : 0.f;
+        run_v += v;  // a run of pix = -1 is never added
       } else {
-        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
+        splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float w) {
           const float vw = v * w;
           if (pix >= 0 && vw != 0.f) add_px(pix, vw);
         });
@@ -562,13 +584,15 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     };
     // two record buffers used alternately (no register copies between
     // iterations); each prefetches the record after the one being traced
-    RecRaw ra = fetch_record(recs + 4 * j0), rb;
-    for (long long j = j0; j < j1; j += 2) {
-      rb = fetch_record(recs + 4 * (j + 1 < j1 ? j + 1 : j));
-      trace_one(ra, j);
-      if (j + 1 >= j1) break;
-      ra = fetch_record(recs + 4 * (j + 2 < j1 ? j + 2 : j + 1));
-      trace_one(rb, j + 1);
+    // the next record is loaded while one is traced (the record table has
+    // padding at its end, so the prefetch needs no clamping)
+    const float4* rp = recs + 4 * (size_t)j0;
+    RecRaw cur = fetch_record(rp);
+    for (int j = j0; j < j1; ++j) {
+      rp += 4;
+      const RecRaw nxt = fetch_record(rp);
+      trace_one(cur, j);
+      cur = nxt;
     }
     if (!SPLAT && run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
   }
@@ -606,6 +630,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
     float* gdst = A.out + (size_t)s * P;
     const double sx = A.sources[3 * s], sy = A.sources[3 * s + 1], sz = A.sources[3 * s + 2];
     const float val = A.values[s];
+    int fe = -1;
+    f3 e = mk3(0.f, 0.f, 0.f);
     for (long long c = wave; c < nchunks; c += wpb) {
       const int f = (int)fdiv((uint32_t)c, A.d_cpf);
       const int m = ((int)c - f * A.cpf) * 64 + lane;
@@ -613,18 +639,23 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
       const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
       const f3 p = mk3(A.world[gi], A.world[total + gi], A.world[2 * total + gi]);
       const f3 n = mk3(A.world[3 * total + gi], A.world[4 * total + gi], A.world[5 * total + gi]);
-      Sample smp = make_sample(A.S, p, n, A.world[6 * total + gi]);
+      const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
+      if (f != fe) {
+        e = source_offset<SRC>(sx, sy, sz, pos);
+        fe = f;
+      }
+      Sample smp = make_sample<KIND>(A.S, p, n, A.world[6 * total + gi], pos);
       smp.f = f;
-      float x, y, v;
-      const bool ok = trace_ray<SRC, true, false>(A, smp, active, sx, sy, sz, val, x, y, v);
+      float gu, gv, v, x, y;
+      const bool ok = trace_ray<SRC, KIND, true, false>(A, smp, active, e, val, gu, gv, v, x, y);
       if (!SPLAT) {
-        const int pix = ok ? hard_pixel<KIND>(A.S, table, x, y) : -1;
+        const int pix = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
         if (A.lds_image)
           wave_accumulate<false>(row, pix, v, A.max_rounds);
         else
           wave_accumulate<true>(gdst, pix, v, A.max_rounds);
       } else {
-        splat_taps<KIND>(A.S, table, x, y, ok, [&](int pix, float w) {
+        splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float w) {
           if (A.lds_image)
             wave_accumulate<false>(row, pix, v * w, A.max_rounds);
           else
@@ -761,8 +792,12 @@ hipError_t launch_cull(const TraceArgs& A, int source_type, float* cones, uint32
 
 hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st) {
   const long long total = (long long)A.nf * A.M;
-  hipLaunchKernelGGL(k_sample_records, dim3(grid_for(total, 256, 16384)), dim3(256), 0, st, A,
-                     recs);
+  if (A.S.kind == IACT_SENSOR_SQUARE)
+    hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_SQUARE>, dim3(grid_for(total, 256, 16384)),
+                       dim3(256), 0, st, A, recs);
+  else
+    hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_HEX>, dim3(grid_for(total, 256, 16384)),
+                       dim3(256), 0, st, A, recs);
   return hipGetLastError();
 }
 

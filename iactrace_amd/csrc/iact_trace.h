@@ -108,9 +108,10 @@ struct TraceArgs {
   int cull_words;
 };
 
-// World-space sample record of the source-major trace (16 words, one scalar
-// load): p as 3 doubles, n, -1/weight, the sensor-plane constants of p, and p
-// in fp32 -- exactly the Sample that make_sample builds.
+// World-space sample record of the source-major trace (16 words, read with
+// broadcast vector loads): q = p - facet position, n, -1/weight, the
+// sensor-grid constants of p, p, and 3 words of padding -- the fields of the
+// Sample that make_sample builds which the image trace uses.
 #define IACT_REC_WORDS 16
 
 // Position of a block in an XCD-major order.  Workgroups are dispatched to
@@ -210,72 +211,100 @@ IACT_DEV bool decode_item(const TraceArgs& A, long long item, int inst, int lane
 // One sample in world space, ready to be traced against many sources.
 struct Sample {
   f3 p, n;
+  f3 q;            // p - facet position, in fp32
   float nrw;       // -1 / weight (the reflected value is value * cos / w, cos = -d.n)
-  double px, py, pz;
   PlaneK K;        // sensor-plane constants of the primary origin
+  GridK G;         // the same in sensor-grid coordinates
   int f;           // facet (selects its obstruction candidate list)
 };
 
-IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w) {
+template <int KIND>
+IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_pos) {
   Sample s;
   s.p = p;
   s.n = n;
+  s.q = p - facet_pos;
   s.nrw = -frcp(w);
-  s.px = (double)p.x;
-  s.py = (double)p.y;
-  s.pz = (double)p.z;
   s.K = plane_consts(S, p);
+  s.G = grid_consts<KIND>(S, s.K);
   s.f = 0;
   return s;
 }
 
+// Primary direction of a ray.  Point sources sit up to ~1e10 m away, so the
+// offset between a facet and a source is formed in fp64 and rounded once,
+// e = fp32(facet position - source), per (facet, source); the ray from a
+// sample p of the facet has direction normalise(q + e), q = p - facet
+// position.  Parallel sources carry their direction, used as given.
+template <int SRC>
+IACT_DEV f3 source_offset(double sx, double sy, double sz, f3 facet_pos) {
+  if (SRC == IACT_SOURCE_POINT)
+    return mk3((float)((double)facet_pos.x - sx), (float)((double)facet_pos.y - sy),
+               (float)((double)facet_pos.z - sz));
+  return mk3((float)sx, (float)sy, (float)sz);
+}
+
+template <int SRC>
+IACT_DEV f3 primary_direction(const Sample& s, f3 e, float& inv) {
+  if (SRC == IACT_SOURCE_POINT) {
+    const f3 u = s.q + e;
+    inv = frsq(fmaf(u.x, u.x, fmaf(u.y, u.y, u.z * u.z)));
+    return mk3(u.x * inv, u.y * inv, u.z * inv);
+  }
+  inv = 1.f;
+  return e;
+}
+
+// Wave-uniform read of a lane's value (v_readlane).
+IACT_DEV float lane_value(float x, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+IACT_DEV f3 lane_value(f3 x, int k) {
+  return mk3(lane_value(x.x, k), lane_value(x.y, k), lane_value(x.z, k));
+}
+
 // Record of a sample (layout documented at IACT_REC_WORDS).
 IACT_DEV void store_record(float* r, const Sample& s) {
-  double* d = (double*)r;
-  d[0] = s.px;
-  d[1] = s.py;
-  d[2] = s.pz;
-  r[6] = s.n.x;
-  r[7] = s.n.y;
-  r[8] = s.n.z;
-  r[9] = s.nrw;
-  r[10] = s.K.kt;
-  r[11] = s.K.kx;
-  r[12] = s.K.ky;
-  r[13] = s.p.x;
-  r[14] = s.p.y;
-  r[15] = s.p.z;
+  r[0] = s.q.x;
+  r[1] = s.q.y;
+  r[2] = s.q.z;
+  r[3] = s.n.x;
+  r[4] = s.n.y;
+  r[5] = s.n.z;
+  r[6] = s.nrw;
+  r[7] = s.G.kt;
+  r[8] = s.G.ku;
+  r[9] = s.G.kv;
+  r[10] = s.p.x;
+  r[11] = s.p.y;
+  r[12] = s.p.z;
+  r[13] = 0.f;
+  r[14] = 0.f;
+  r[15] = 0.f;
 }
 
-// Record as four 16-byte vector loads (every lane reads the same address:
-// one broadcast line).  Vector loads complete in order, so a load issued one
-// sample ahead is waited for on its own.
+// Record read with scalar loads (the address is wave-uniform): 13 words into
+// SGPRs.  Vector loads of a uniform address would return the same 52 bytes
+// to each of the 64 lanes, and that data return, not the arithmetic, bounds
+// the trace; SGPR operands cost a move only where an instruction needs two.
 struct RecRaw {
-  float4 a, b, c, d;
+  float4 a, b, c;
+  float d;
 };
-IACT_DEV RecRaw fetch_record(const float4* r) { return RecRaw{r[0], r[1], r[2], r[3]}; }
+IACT_DEV RecRaw fetch_record(const float4* r) {
+  const cptr<float> q = as_const((const float*)r);
+  return RecRaw{make_float4(q[0], q[1], q[2], q[3]), make_float4(q[4], q[5], q[6], q[7]),
+                make_float4(q[8], q[9], q[10], q[11]), q[12]};
+}
 IACT_DEV Sample unpack_record(const RecRaw& q) {
   Sample s;
-  s.px = __hiloint2double(__float_as_int(q.a.y), __float_as_int(q.a.x));
-  s.py = __hiloint2double(__float_as_int(q.a.w), __float_as_int(q.a.z));
-  s.pz = __hiloint2double(__float_as_int(q.b.y), __float_as_int(q.b.x));
-  s.n = mk3(q.b.z, q.b.w, q.c.x);
-  s.nrw = q.c.y;
-  s.K = PlaneK{q.c.z, q.c.w, q.d.x};
-  s.p = mk3(q.d.y, q.d.z, q.d.w);
-  return s;
-}
-
-// Wave-uniform (scalar) load of a record.
-IACT_DEV Sample load_record(cptr<uint32_t> r) {
-  Sample s;
-  s.px = __hiloint2double((int)r[1], (int)r[0]);
-  s.py = __hiloint2double((int)r[3], (int)r[2]);
-  s.pz = __hiloint2double((int)r[5], (int)r[4]);
-  s.n = mk3(__uint_as_float(r[6]), __uint_as_float(r[7]), __uint_as_float(r[8]));
-  s.nrw = __uint_as_float(r[9]);
-  s.K = PlaneK{__uint_as_float(r[10]), __uint_as_float(r[11]), __uint_as_float(r[12])};
-  s.p = mk3(__uint_as_float(r[13]), __uint_as_float(r[14]), __uint_as_float(r[15]));
+  s.q = mk3(q.a.x, q.a.y, q.a.z);
+  s.n = mk3(q.a.w, q.b.x, q.b.y);
+  s.nrw = q.b.z;
+  s.G = GridK{q.b.w, q.c.x, q.c.y};
+  s.p = mk3(q.c.z, q.c.w, q.d);
+  s.K = PlaneK{0.f, 0.f, 0.f};  // world hit coordinates are not traced from records
+  s.f = 0;
   return s;
 }
 
@@ -409,55 +438,82 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
   stage_forward<false>(A, stage, o, d, v, active, unused);
 }
 
+// Shadow factor of the primary leg: obstructions between the sample and
+// the source (render.py:118-128), from the facet's candidate list when the
+// launch culls.  OBST: 0 = the scene has no obstructions, 1 = it has, 2 =
+// test at run time.
+template <int OBST>
+IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool active) {
+  if (!(OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))) return 1.f;
+  return A.cull ? shadow_factor_culled(A.ob, as_const(A.cull) + (size_t)s.f * A.cull_words,
+                                       A.cull_words, s.p, -d, active)
+                : shadow_factor(A.ob, s.p, -d, active);
+}
+
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
-// Returns false when the ray misses the plane (the reference's 1e10
-// sentinel) or the lane is inactive; v is 0 for inactive lanes.
-// OBST: 0 = the scene has no obstructions, 1 = it has, 2 = test at run time.
-template <int SRC, bool STAGES, bool AXIS, int OBST = 2>
-IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, double sx, double sy,
-                        double sz, float value, float& x, float& y, float& v) {
-  f3 d;
-  if (SRC == IACT_SOURCE_POINT) {
-    // p - src in float64 (sources sit at ~1e10 m), then normalised in fp32.
-    const float ddx = (float)(s.px - sx), ddy = (float)(s.py - sy), ddz = (float)(s.pz - sz);
-    const float inv = frsq(fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz)));
-    d = mk3(ddx * inv, ddy * inv, ddz * inv);
-  } else {
-    d = mk3((float)sx, (float)sy, (float)sz);  // used as given (Q15)
-  }
+// e is the facet's source offset (source_offset).  Returns false when the
+// ray misses the plane (the reference's 1e10 sentinel) or the lane is
+// inactive; v is 0 for inactive lanes.  (gu, gv) is the hit in sensor-grid
+// coordinates (what the binning reads); XY also gives the hit (x, y) in the
+// sensor frame.
+template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false>
+IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float value,
+                        float& gu, float& gv, float& v, float& x, float& y) {
+  float inv;
+  const f3 d = primary_direction<SRC>(s, e, inv);
   const float c = dot(d, s.n);
   f3 r = fma3(-2.f * c, s.n, d);
   v = value * c * s.nrw;
-  if (OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))
-    v *= A.cull ? shadow_factor_culled(A.ob, as_const(A.cull) + (size_t)s.f * A.cull_words,
-                                       A.cull_words, s.p, -d, active)
-                : shadow_factor(A.ob, s.p, -d, active);
+  v *= primary_shadow<OBST>(A, s, d, active);
   PlaneK K = s.K;
+  GridK G = s.G;
   if (STAGES && A.n_stages > 0) {
     f3 o = s.p;
     for (int k = 0; k < A.n_stages; ++k) reflect_at_stage(A, A.stages[k], o, r, v, active);
     K = plane_consts(A.S, o);
+    G = grid_consts<KIND>(A.S, K);
   }
-  const bool ok = plane_xy<AXIS>(A.S, K, r, x, y);
+  const bool ok = plane_grid<KIND, AXIS>(A.S, G, r, gu, gv);
+  if (XY) plane_xy<AXIS>(A.S, K, r, x, y);
   if (!active) v = 0.f;
   return ok && active;
 }
 
+// Hard pixel of a hit in sensor-grid coordinates; -1 when dropped.
+template <int KIND>
+IACT_DEV int grid_pixel(const DevSensor& S, const int32_t* table, float gu, float gv) {
+  if (KIND == IACT_SENSOR_SQUARE) return square_pixel_grid(S, gu, gv);
+  return hex_pixel_grid(S, table, gu, gv);
+}
+
+// Hard pixel of a hit (x, y) in the sensor frame.
 template <int KIND>
 IACT_DEV int hard_pixel(const DevSensor& S, const int32_t* table, float x, float y) {
   if (KIND == IACT_SENSOR_SQUARE) return square_pixel(S, x, y);
   return hex_pixel(S, table, x, y);
 }
 
+// Sensor-frame (x, y) -> sensor-grid coordinates.
+template <int KIND>
+IACT_DEV void to_grid(const DevSensor& S, float x, float y, float& gu, float& gv) {
+  if (KIND == IACT_SENSOR_SQUARE) {
+    gu = fmaf(x, S.inv_dx, S.nx0);
+    gv = fmaf(y, S.inv_dy, S.ny0);
+  } else {
+    hex_axial(S, x, y, gu, gv);
+  }
+}
+
 // Splat taps of one ray (square.py:144-173, hexagonal.py:264-315).  Calls
 // fn(pixel, weight) for every tap; weights are normalised to sum 1.  Rays
 // that miss the plane or whose normaliser underflows contribute nothing.
+// The hit is given in sensor-grid coordinates (gu, gv).
 template <int KIND, typename Fn>
-IACT_DEV void splat_taps(const DevSensor& S, const int32_t* table, float x, float y, bool ok_in,
-                         Fn&& fn) {
+IACT_DEV void splat_taps(const DevSensor& S, const int32_t* table, float gu, float gv,
+                         bool ok_in, Fn&& fn) {
   const int ks = S.ks;
   if (KIND == IACT_SENSOR_SQUARE) {
-    const float xp = fmaf(x, S.inv_dx, S.nx0), yp = fmaf(y, S.inv_dy, S.ny0);
+    const float xp = gu, yp = gv;
     const float xb = floorf(xp), yb = floorf(yp);
     const float xf = xp - xb, yf = yp - yb;
     float tot = 0.f;
@@ -474,11 +530,11 @@ IACT_DEV void splat_taps(const DevSensor& S, const int32_t* table, float x, floa
         const float w = __expf(S.neg_half_inv_sigma2 * fmaf(ex, ex, ey * ey)) * it;
         const float xi = xb + (float)ox, yi = yb + (float)oy;
         const bool in = ok && xi >= 0.f && xi < S.wf && yi >= 0.f && yi < S.hf;
-        fn(in ? (int)yi * S.width + (int)xi : -1, w);
+        fn(in ? (int)fmaf(yi, S.wf, xi) : -1, w);
       }
   } else {
-    float q, r, qb, rb;
-    hex_axial(S, x, y, q, r);
+    const float q = gu, r = gv;
+    float qb, rb;
     hex_round(q, r, qb, rb);
     const float dq = q - qb, dr = r - rb;
     float tot = 0.f;
