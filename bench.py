@@ -106,7 +106,7 @@ def c3_fwd_bwd(ia, configs, steps, warmup):
 
 def c3t_culling(ia, configs, steps, warmup):
     """C3 with a CT5-sized support truss (271 obstructions): forward rays/s
-    with obstruction culling (the default from 16 obstructions) and without."""
+    with obstruction culling (the default) and without."""
     import torch
     dev = torch.device("cuda:0")
     wl = configs.workload("C3T")

@@ -331,16 +331,9 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
       (lay == 2 || (lay == 0 && ns >= 64)))
     A.layout = 2;
   // obstruction culling
-  const int cm = L ? L->culling : 0;
-  if (cm < -1 || cm > 1) return set_error(IACT_ERR_INVALID, "culling mode %d", cm);
-  const int nob = n_obstructions(sc.obstructions);
-  const bool fits = (long long)A.nf * A.M < (1ll << 32);
-  if (cm == 1 && !fits)
-    return set_error(IACT_ERR_UNSUPPORTED, "culling needs facets x samples < 2^32");
   A.cull = nullptr;
-  A.cull_words = (nob > 0 && fits && (cm == 1 || (cm == 0 && nob >= kAutoCullObstructions)))
-                     ? (nob + 31) / 32
-                     : 0;
+  int rc = cull_words_for(sc, L, A.cull_words);
+  if (rc) return rc;
   A.d_m = make_fastdiv((uint32_t)(A.M > 0 ? A.M : 1));
   if (bpc == 0) bpc = default_blocks_per_cu(A.det);
   const long long cap = (long long)dev.n_cu * bpc;
@@ -404,6 +397,15 @@ static size_t base_workspace_bytes(const iact_scene& sc, int mode, const LaunchS
 // cull_words mask words per facet.
 static size_t cull_offset(size_t base) { return (base + 255) & ~(size_t)255; }
 
+int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words) {
+  const int cm = L ? L->culling : 0;
+  if (cm < -1 || cm > 1) return set_error(IACT_ERR_INVALID, "culling mode %d", cm);
+  const int nob = n_obstructions(sc.obstructions);
+  words = (nob > 0 && (cm == 1 || (cm == 0 && nob >= kAutoCullObstructions))) ? (nob + 31) / 32
+                                                                              : 0;
+  return IACT_OK;
+}
+
 size_t cull_bytes(const TraceArgs& A) {
   return A.cull_words > 0 ? (size_t)A.nf * (8 + A.cull_words) * sizeof(float) : 0;
 }
@@ -415,11 +417,15 @@ size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchS
   return A.cull_words > 0 ? cull_offset(base) + cull_bytes(A) : base;
 }
 
-int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st) {
+int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
+                 long long n_cone_sources) {
   if (A.cull_words == 0 || A.nf == 0 || A.ns == 0) return IACT_OK;
   float* cones = (float*)((char*)workspace + cull_offset(base));
   uint32_t* masks = (uint32_t*)(cones + 8 * (size_t)A.nf);
-  const hipError_t e = launch_cull(A, source_type, cones, masks, st);
+  // the cones cover A.sources[0, n_cone_sources) (every instance of a batch)
+  TraceArgs C = A;
+  if (n_cone_sources >= 0) C.ns = (int)n_cone_sources;
+  const hipError_t e = launch_cull(C, source_type, cones, masks, st);
   if (e != hipSuccess) return hip_error(e, "k_facet_cones/k_cull_masks");
   A.cull = masks;
   return IACT_OK;

@@ -19,12 +19,15 @@ int default_blocks_per_cu(int det);
 // size; updates the work order that depends on the grid.
 int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShape& shp,
            TraceArgs& A);
-// Obstruction culling: automatic from this many obstructions; tables of
+// Obstruction culling: automatic from this many obstructions (a dish with a
+// camera box and four masts already traces 2.3x faster culled); tables of
 // cull_bytes() follow a mode's own workspace of `base` bytes, filled by
 // prepare_cull (which also points A.cull at the masks).
-constexpr int kAutoCullObstructions = 16;
+constexpr int kAutoCullObstructions = 1;
+int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words);
 size_t cull_bytes(const TraceArgs& A);
-int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st);
+int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
+                 long long n_cone_sources = -1);
 int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* where);
 int validate_scene(const iact_scene* sc);

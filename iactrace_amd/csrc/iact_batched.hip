@@ -122,38 +122,88 @@ static int validate_batch(const iact_batch* b, long long& total, int& max_ns) {
   return IACT_OK;
 }
 
-// Shared set-up of the batched forward and backward: trace arguments with
-// the batch fields, the launch shape and the item count from the host offsets.
-static int prepare_batch(const iact_scene* scene, const iact_batch* batch, int source_type,
-                         const iact_launch* launch, void* workspace, size_t workspace_bytes,
-                         TraceArgs& A, LaunchShape& shp, bool& empty,
-                         const int64_t*& src_prefix, iact_launch& Lout, DevInfo& devout) {
+// Launch shape of a batch (no workspace needed): trace arguments, shape and
+// the culling decision.  The facet cones of a batch cover the sources of
+// every instance (a cone only has to contain the rays it bounds, and the
+// instances' samples stay within the same distance of their facet
+// positions), so one candidate table serves all instances.
+static int batch_shape(const iact_scene* scene, const iact_batch* batch, int source_type,
+                       const iact_launch* launch, TraceArgs& A, LaunchShape& shp,
+                       iact_launch& L, DevInfo& dev, long long& total) {
+  int rc = validate_scene(scene);
+  if (rc) return rc;
+  int max_ns;
+  rc = validate_batch(batch, total, max_ns);
+  if (rc) return rc;
+  if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL &&
+      source_type != -1)
+    return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
+  if (launch && launch->deterministic)
+    return set_error(IACT_ERR_UNSUPPORTED, "batched renders have no deterministic mode");
+  rc = device_info(dev);
+  if (rc) return rc;
+  fill_trace_args(A, *scene, batch->sources, batch->values, max_ns);
+  // sample-major layout, wave-merged accumulation into a per-block LDS image
+  // of the instance being traced (no run-length or deterministic variants)
+  L = launch ? *launch : iact_launch{};
+  L.accumulate = 0;
+  L.layout = 1;
+  rc = shape_trace(*scene, max_ns, IACT_MODE_IMAGE, &L, A, shp, dev, -1);
+  if (rc) return rc;
+  if (total >= (1ll << 31)) {
+    if (L.culling == 1)
+      return set_error(IACT_ERR_UNSUPPORTED, "culling a batch of 2^31 or more sources");
+    A.cull_words = 0;
+  }
+  return IACT_OK;
+}
+
+// Workspace of a batch: prefix and instance table, then the candidate
+// tables when the batch culls (known without a device).
+static int batch_need(const iact_scene* scene, const iact_batch* batch, const iact_launch* launch,
+                      size_t& need) {
   int rc = validate_scene(scene);
   if (rc) return rc;
   long long total;
   int max_ns;
   rc = validate_batch(batch, total, max_ns);
   if (rc) return rc;
+  int words = 0;
+  rc = cull_words_for(*scene, launch, words);
+  if (rc) return rc;
+  if (total >= (1ll << 31)) words = 0;
+  const size_t c = words ? (size_t)scene->facets.n_facets * (8 + words) * sizeof(float) : 0;
+  need = batch_ws_bytes(batch->n_instances) + align256(c);
+  return IACT_OK;
+}
+
+// Shared set-up of the batched forward and backward: trace arguments with
+// the batch fields, the launch shape and the item count from the host
+// offsets, and the obstruction candidate tables.
+static int prepare_batch(const iact_scene* scene, const iact_batch* batch, int source_type,
+                         const iact_launch* launch, void* workspace, size_t workspace_bytes,
+                         TraceArgs& A, LaunchShape& shp, bool& empty,
+                         const int64_t*& src_prefix, iact_launch& Lout, DevInfo& devout,
+                         hipStream_t st, bool tables = true) {
+  iact_launch L;
+  DevInfo dev;
+  long long total;
   if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL)
     return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
   if (launch && launch->deterministic)
     return set_error(IACT_ERR_UNSUPPORTED, "batched renders have no deterministic mode");
-  const size_t need = batch_ws_bytes(batch->n_instances);
+  size_t need = 0;
+  int rc = batch_need(scene, batch, launch, need);
+  if (rc) return rc;
   if (!workspace || workspace_bytes < need)
     return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
                      workspace_bytes);
-  DevInfo dev;
-  rc = device_info(dev);
+  rc = batch_shape(scene, batch, source_type, launch, A, shp, L, dev, total);
   if (rc) return rc;
-  fill_trace_args(A, *scene, batch->sources, batch->values, max_ns);
-  // sample-major layout, wave-merged accumulation into a per-block LDS image
-  // of the instance being traced (no run-length or deterministic variants)
-  iact_launch L = launch ? *launch : iact_launch{};
-  L.accumulate = 0;
-  L.layout = 1;
-  L.culling = -1;  // the facet cones would depend on each instance's sources
-  rc = shape_trace(*scene, max_ns, IACT_MODE_IMAGE, &L, A, shp, dev, -1);
-  if (rc) return rc;
+  if (tables && total > 0) {
+    rc = prepare_cull(A, workspace, batch_ws_bytes(batch->n_instances), source_type, st, total);
+    if (rc) return rc;
+  }
   const int wpb = shp.threads / 64;
   const long long nchunks = (long long)A.nf * A.cpf;
   long long items = 0;
@@ -198,9 +248,10 @@ using namespace iact;
 
 extern "C" {
 
-size_t iact_batch_workspace_size(const iact_scene*, const iact_batch* batch, const iact_launch*) {
-  if (!batch || batch->n_instances < 1) return 0;
-  return batch_ws_bytes(batch->n_instances);
+size_t iact_batch_workspace_size(const iact_scene* scene, const iact_batch* batch,
+                                 const iact_launch* launch) {
+  size_t need = 0;
+  return batch_need(scene, batch, launch, need) ? 0 : need;
 }
 
 int32_t iact_trace_batched_fwd(const iact_scene* scene, const iact_batch* batch,
@@ -213,7 +264,7 @@ int32_t iact_trace_batched_fwd(const iact_scene* scene, const iact_batch* batch,
   iact_launch L;
   DevInfo dev;
   int rc = prepare_batch(scene, batch, source_type, launch, workspace, workspace_bytes, A, shp,
-                         empty, sp, L, dev);
+                         empty, sp, L, dev, (hipStream_t)stream);
   if (rc) return rc;
   if (!images) return set_error(IACT_ERR_INVALID, "null images");
   hipStream_t st = (hipStream_t)stream;
@@ -238,7 +289,7 @@ int32_t iact_trace_batched_bwd(const iact_scene* scene, const iact_batch* batch,
   iact_launch L;
   DevInfo dev;
   int rc = prepare_batch(scene, batch, source_type, launch, workspace, workspace_bytes, A, shp,
-                         empty, sp, L, dev);
+                         empty, sp, L, dev, (hipStream_t)stream);
   if (rc) return rc;
   if (!grad_images || !grad_rotations)
     return set_error(IACT_ERR_INVALID, "null image or rotation gradients");
@@ -323,7 +374,7 @@ int32_t iact_fit_graph_create(const iact_scene* scene, const iact_batch* batch,
   iact_launch L;
   DevInfo dev;
   int rc = prepare_batch(scene, batch, source_type, launch, workspace, workspace_bytes, A, shp,
-                         empty, sp, L, dev);
+                         empty, sp, L, dev, nullptr, false);
   if (rc) return rc;
   if (scene->n_stages > 0)
     return set_error(IACT_ERR_UNSUPPORTED, "fit graphs cover primary-only scenes");

@@ -166,7 +166,7 @@ typedef struct iact_launch {
                               keeps the obstructions it can reach (a bit
                               mask per facet in the workspace); shadows of
                               the other obstructions are not tested.  0 =
-                              automatic (on from 16 obstructions), 1 = on,
+                              automatic (on with any obstruction), 1 = on,
                               -1 = off.  Results are identical either way. */
 } iact_launch;
 

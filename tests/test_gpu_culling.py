@@ -148,3 +148,33 @@ def test_culling_mode_validation():
     with pytest.raises(ia._lib.IactError):
         ia.render(tel, torch.as_tensor(wl.sources), torch.as_tensor(wl.values), "point",
                   launch=ia.LaunchConfig(culling=2))
+
+
+def test_batched_culling_matches_unculled():
+    """One candidate table serves every instance of a batch (cones over all
+    the batch's sources); images and rotation gradients must not change."""
+    wl = configs.workload("C3T", n_sources=8, n_samples=64)
+    tel = wl.telescope(DEV, key=7)
+    base = tel.mirror_groups[0].rotations.detach()
+    rng = np.random.default_rng(3)
+    n_inst = 3
+    rot = base.unsqueeze(0).repeat(n_inst, 1, 1).clone()
+    rot[:, :, :2] += torch.as_tensor(rng.normal(0, 0.05, (n_inst, base.shape[0], 2)),
+                                     dtype=torch.float32, device=DEV)
+    src, vals = [], []
+    for i, n in enumerate((40, 0, 70)):
+        a, b = configs.point_source_field(n, 2.0, seed=80 + i)
+        src.append(a)
+        vals.append(b)
+    s, v, off = ia.pack_sources(src, vals)
+    W = torch.linspace(0.5, 1.5, tel.sensors[0].n_pixels, device=DEV)
+    out = []
+    for L in (ON, OFF):
+        r = rot.clone().requires_grad_(True)
+        img = ia.render_batched(tel, r, s, v, off, "point", launch=L)
+        (img * W).sum().backward()
+        out.append((img.detach(), r.grad.double()))
+    ei = (out[0][0] - out[1][0]).abs().max().item() / out[1][0].abs().max().item()
+    eg = (out[0][1] - out[1][1]).abs().max().item() / out[1][1].abs().max().item()
+    print(f"[batched-culling] image rel err {ei:.2e}, gradient rel err {eg:.2e}", flush=True)
+    assert ei < 1e-5 and eg < 1e-5
