@@ -332,7 +332,7 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
     A.layout = 2;
   // obstruction culling
   A.cull = nullptr;
-  int rc = cull_words_for(sc, L, A.cull_words);
+  int rc = cull_words_for(sc, L, A.cull_words, A.cull_all);
   if (rc) return rc;
   A.d_m = make_fastdiv((uint32_t)(A.M > 0 ? A.M : 1));
   if (bpc == 0) bpc = default_blocks_per_cu(A.det);
@@ -397,12 +397,12 @@ static size_t base_workspace_bytes(const iact_scene& sc, int mode, const LaunchS
 // cull_words mask words per facet.
 static size_t cull_offset(size_t base) { return (base + 255) & ~(size_t)255; }
 
-int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words) {
+int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all) {
   const int cm = L ? L->culling : 0;
   if (cm < -1 || cm > 1) return set_error(IACT_ERR_INVALID, "culling mode %d", cm);
   const int nob = n_obstructions(sc.obstructions);
-  words = (nob > 0 && (cm == 1 || (cm == 0 && nob >= kAutoCullObstructions))) ? (nob + 31) / 32
-                                                                              : 0;
+  words = nob > 0 ? (nob + 31) / 32 : 0;
+  all = !(cm == 1 || (cm == 0 && nob >= kAutoCullObstructions));
   return IACT_OK;
 }
 
@@ -425,7 +425,7 @@ int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hi
   // the cones cover A.sources[0, n_cone_sources) (every instance of a batch)
   TraceArgs C = A;
   if (n_cone_sources >= 0) C.ns = (int)n_cone_sources;
-  const hipError_t e = launch_cull(C, source_type, cones, masks, st);
+  const hipError_t e = launch_cull(C, source_type, A.cull_all, cones, masks, st);
   if (e != hipSuccess) return hip_error(e, "k_facet_cones/k_cull_masks");
   A.cull = masks;
   return IACT_OK;

@@ -20,11 +20,12 @@ int default_blocks_per_cu(int det);
 int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShape& shp,
            TraceArgs& A);
 // Obstruction culling: automatic from this many obstructions (a dish with a
-// camera box and four masts already traces 2.3x faster culled); tables of
-// cull_bytes() follow a mode's own workspace of `base` bytes, filled by
-// prepare_cull (which also points A.cull at the masks).
+// camera box and four masts already traces 2.3x faster culled).  Every scene
+// with obstructions gets candidate masks (all bits set when not culling);
+// the tables of cull_bytes() follow a mode's own workspace of `base` bytes
+// and are filled by prepare_cull (which also points A.cull at the masks).
 constexpr int kAutoCullObstructions = 1;
-int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words);
+int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all);
 size_t cull_bytes(const TraceArgs& A);
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
                  long long n_cone_sources = -1);

@@ -507,6 +507,7 @@ const void* pick_adjoint_kernel(const AdjArgs& B, int source_type) {
 
 hipError_t launch_adjoint(const AdjArgs& B, const LaunchShape& L, int source_type,
                           hipStream_t st) {
+  if (has_obstructions(B.T.ob) && !B.T.cull) return hipErrorInvalidValue;
   AdjArgs b = B;
   void* args[] = {&b};
   return hipLaunchKernel(pick_adjoint_kernel(B, source_type), dim3(L.grid), dim3(L.threads),

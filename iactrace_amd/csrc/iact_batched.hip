@@ -153,7 +153,7 @@ static int batch_shape(const iact_scene* scene, const iact_batch* batch, int sou
   if (total >= (1ll << 31)) {
     if (L.culling == 1)
       return set_error(IACT_ERR_UNSUPPORTED, "culling a batch of 2^31 or more sources");
-    A.cull_words = 0;
+    A.cull_all = 1;  // the cone pass counts sources in 32 bits
   }
   return IACT_OK;
 }
@@ -168,10 +168,9 @@ static int batch_need(const iact_scene* scene, const iact_batch* batch, const ia
   int max_ns;
   rc = validate_batch(batch, total, max_ns);
   if (rc) return rc;
-  int words = 0;
-  rc = cull_words_for(*scene, launch, words);
+  int words = 0, all = 0;
+  rc = cull_words_for(*scene, launch, words, all);
   if (rc) return rc;
-  if (total >= (1ll << 31)) words = 0;
   const size_t c = words ? (size_t)scene->facets.n_facets * (8 + words) * sizeof(float) : 0;
   need = batch_ws_bytes(batch->n_instances) + align256(c);
   return IACT_OK;

@@ -404,6 +404,11 @@ IACT_DEV float shadow_factor_culled(const iact_obstructions& ob, cptr<uint32_t> 
   const float big = 1e10f;
   const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
   const int c3 = c2 + ob.n_oriented_boxes;
+  // most facets of a dish have no candidate at all: leave before the
+  // per-ray set-up that the compiler hoists out of the tests
+  uint32_t any = 0u;
+  for (int w = 0; w < words; ++w) any |= mask[w];
+  if (any == 0u) return 1.f;
   bool blocked = !active;
   for (int w = 0; w < words; ++w) {
     uint32_t bits = mask[w];

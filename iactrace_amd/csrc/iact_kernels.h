@@ -31,8 +31,8 @@ hipError_t launch_accumulate(const DevSensor& S, const float* x, const float* y,
 hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_type,
                            hipStream_t st);
 hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st);
-hipError_t launch_cull(const TraceArgs& A, int source_type, float* cones, uint32_t* masks,
-                       hipStream_t st);
+hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cones,
+                       uint32_t* masks, hipStream_t st);
 hipError_t launch_reduce(const float* partials, int nblocks, int P, float* out, hipStream_t st);
 
 // Arguments of the adjoint kernel (iact_adjoint.hip).

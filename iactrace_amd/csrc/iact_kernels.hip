@@ -240,7 +240,7 @@ IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
 // Candidate bits: one thread per (facet, mask word).  Long cylinders are
 // tested as a chain of bounding spheres of pieces at most 1 m long (at most
 // 64 pieces).
-__global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks) {
+__global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, int all) {
   const iact_obstructions& ob = A.ob;
   const int words = A.cull_words;
   const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
@@ -250,13 +250,13 @@ __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks) {
        id += (long long)gridDim.x * blockDim.x) {
     const int f = (int)(id / words), w = (int)(id - (long long)f * words);
     const float* cn = cones + 8 * (size_t)f;
-    const f3 c = mk3(cn[0], cn[1], cn[2]);
-    const float rho = cn[3], tn = cn[7];
-    const f3 a = mk3(cn[4], cn[5], cn[6]);
+    const f3 c = all ? mk3(0.f, 0.f, 0.f) : mk3(cn[0], cn[1], cn[2]);
+    const float rho = all ? 0.f : cn[3], tn = all ? -1.f : cn[7];
+    const f3 a = all ? mk3(0.f, 0.f, 1.f) : mk3(cn[4], cn[5], cn[6]);
     uint32_t bits = 0u;
     for (int j = 32 * w; j < min(32 * w + 32, c4); ++j) {
       bool hit;
-      if (tn < 0.f) {
+      if (all || tn < 0.f) {
         hit = true;
       } else if (j < c0) {
         const float* q = ob.cylinders + 8 * j;
@@ -352,16 +352,22 @@ hipError_t launch_accumulate(const DevSensor& S, const float* x, const float* y,
 // ------------------------------------------------------------------ trace
 // Image / debug mode.  Work item = (source tile, chunk of 64 samples of one
 // facet); items are distributed over the waves of a persistent grid.
-template <int SRC, int KIND, int SPLAT, int MODE, bool STAGES, bool AXIS>
+// LEAN: the launch accumulates into one LDS image per block with wave-merged
+// adds (the batched forward and the default sample-major image); the other
+// accumulation paths are compiled out, which keeps scalar registers free.
+template <int SRC, int KIND, int SPLAT, int MODE, bool STAGES, bool AXIS, bool LEAN = false>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   extern __shared__ float smem[];
+  const bool lds_image = LEAN || A.lds_image;
+  const bool det = !LEAN && A.det;
+  const int acc = LEAN ? 0 : A.acc;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
   const int lane = lane_id();
   const int P = A.S.n_pixels;
   float* img_lds = smem;
-  int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? (A.det ? wpb : 1) * P : 0));
-  if (MODE == IACT_MODE_IMAGE && A.lds_image) {
-    const int n = (A.det ? wpb : 1) * P;
+  int32_t* tab_lds = (int32_t*)(smem + (lds_image ? (det ? wpb : 1) * P : 0));
+  if (MODE == IACT_MODE_IMAGE && lds_image) {
+    const int n = (det ? wpb : 1) * P;
     for (int i = threadIdx.x; i < n; i += blockDim.x) img_lds[i] = 0.f;
   }
   if (KIND == IACT_SENSOR_HEX && A.lds_table) {
@@ -370,7 +376,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   }
   __syncthreads();
   const int32_t* table = tab_lds;  // hex tables are always staged in LDS
-  float* img = img_lds + (A.det ? wave * P : 0);
+  float* img = img_lds + (det ? wave * P : 0);
 
   int run_pix = -1;
   float run_v = 0.f;
@@ -381,7 +387,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   // instance; everything down to `continue` is block-uniform
   for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += gridDim.x) {
     const int inst = item_instance(A, bi * wpb);
-    if (MODE == IACT_MODE_IMAGE && A.lds_image && inst != cur) {
+    if (MODE == IACT_MODE_IMAGE && lds_image && inst != cur) {
       __syncthreads();
       for (int i = threadIdx.x; i < P; i += blockDim.x) {
         const float v = img_lds[i];
@@ -434,7 +440,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
         }
       } else if (!SPLAT) {
         const int pix = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
-        if (A.lds_image && A.acc == 1) {
+        if (lds_image && acc == 1) {
           // per-lane run of equal pixels, flushed when the pixel changes
           if (pix != run_pix) {
             if (run_pix >= 0 && run_v != 0.f) atomicAdd(img + run_pix, run_v);
@@ -442,16 +448,16 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
             run_v = 0.f;
           }
           run_v += v;  // a run of pix = -1 is never added
-        } else if (A.lds_image && A.acc == 2) {
+        } else if (lds_image && acc == 2) {
           if (pix >= 0 && v != 0.f) atomicAdd(img + pix, v);
-        } else if (A.lds_image) {
+        } else if (lds_image) {
           wave_accumulate<false>(img, pix, v, A.max_rounds);
         } else {
           wave_accumulate<true>(gout, pix, v, A.max_rounds);
         }
       } else {
         splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float wt) {
-          if (A.lds_image)
+          if (lds_image)
             wave_accumulate<false>(img, pix, v * wt, A.max_rounds);
           else
             wave_accumulate<true>(gout, pix, v * wt, A.max_rounds);
@@ -460,10 +466,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     }
   }
 
-  if (MODE == IACT_MODE_IMAGE && A.lds_image) {
-    if (!SPLAT && A.acc == 1 && run_pix >= 0 && run_v != 0.f) atomicAdd(img + run_pix, run_v);
+  if (MODE == IACT_MODE_IMAGE && lds_image) {
+    if (!SPLAT && acc == 1 && run_pix >= 0 && run_v != 0.f) atomicAdd(img + run_pix, run_v);
     __syncthreads();
-    if (A.det) {
+    if (det) {
       for (int i = threadIdx.x; i < P; i += blockDim.x) {
         float s = 0.f;
         for (int w = 0; w < wpb; ++w) s += img_lds[w * P + i];
@@ -711,9 +717,13 @@ static TraceKernel pick_trace_t(const TraceArgs& A) {
   // image mode is specialised for primary-only scenes and identity sensor
   // frames; both specialisations compute bit-identical results to the
   // generic kernel, they only skip instructions
-  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0)
+  if (MODE == IACT_MODE_IMAGE && A.n_stages == 0) {
+    if (A.lds_image && A.acc == 0 && !A.det)
+      return A.S.axis ? k_trace<SRC, KIND, SPLAT, MODE, false, true, true>
+                      : k_trace<SRC, KIND, SPLAT, MODE, false, false, true>;
     return A.S.axis ? k_trace<SRC, KIND, SPLAT, MODE, false, true>
                     : k_trace<SRC, KIND, SPLAT, MODE, false, false>;
+  }
   return k_trace<SRC, KIND, SPLAT, MODE, true, false>;
 }
 
@@ -760,6 +770,8 @@ TraceKernel pick_trace_kernel(const TraceArgs& A, int source_type, int which) {
 
 hipError_t launch_trace_kernel(TraceKernel k, const TraceArgs& A, const LaunchShape& L,
                                hipStream_t st) {
+  // the primary-leg shadow reads the candidate masks of any obstructed scene
+  if (has_obstructions(A.ob) && !A.cull) return hipErrorInvalidValue;
   TraceArgs a = A;
   void* args[] = {&a};
   return hipLaunchKernel((const void*)k, dim3(L.grid), dim3(L.threads), args, L.lds, st);
@@ -778,15 +790,17 @@ hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_
   return launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR), A, L, st);
 }
 
-hipError_t launch_cull(const TraceArgs& A, int source_type, float* cones, uint32_t* masks,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_facet_cones, dim3(grid_for(A.nf, 1, 4096)), dim3(256), 0, st, A,
-                     source_type, cones);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cones,
+                       uint32_t* masks, hipStream_t st) {
+  if (!all) {
+    hipLaunchKernelGGL(k_facet_cones, dim3(grid_for(A.nf, 1, 4096)), dim3(256), 0, st, A,
+                       source_type, cones);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const long long total = (long long)A.nf * A.cull_words;
   hipLaunchKernelGGL(k_cull_masks, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, A, cones,
-                     masks);
+                     masks, all);
   return hipGetLastError();
 }
 

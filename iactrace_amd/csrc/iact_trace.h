@@ -103,9 +103,10 @@ struct TraceArgs {
   FastDiv d_m;                   // divide by M (sample -> facet)
   long long sm_items;            // sample groups * source chunks
   // per-facet obstruction candidate bit masks for the primary leg's shadow
-  // rays (NULL: every obstruction is tested)
+  // rays; set whenever the scene has obstructions (culling off = every bit)
   const uint32_t* cull;
   int cull_words;
+  int cull_all;                  // masks with every obstruction (culling off)
 };
 
 // World-space sample record of the source-major trace (16 words, read with
@@ -199,12 +200,18 @@ IACT_DEV bool decode_item(const TraceArgs& A, long long item, int inst, int lane
   const uint32_t cbs = Db.d;
   const uint32_t chunk = first + (rem - stile * cbs);
   const uint32_t f = fdiv(chunk, A.d_cpf);
-  w.inst = inst;
-  w.sbase = I.sbase;
-  w.f = (int)f;
+  // everything but the lane's sample is wave-uniform; readfirstlane states it
+  // for the compiler, which otherwise may keep these in vector registers in
+  // large kernels (then the source loop and its loads turn vector too)
+  w.inst = __builtin_amdgcn_readfirstlane(inst);
+  w.sbase = (long long)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                             (int)(uint32_t)((uint64_t)I.sbase >> 32))
+                         << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)I.sbase));
+  w.f = __builtin_amdgcn_readfirstlane((int)f);
   w.m = (int)(chunk - f * A.d_cpf.d) * 64 + lane;
-  w.s0 = (int)stile * A.source_tile;
-  w.s1 = min(w.s0 + A.source_tile, I.ns);
+  w.s0 = __builtin_amdgcn_readfirstlane((int)stile * A.source_tile);
+  w.s1 = __builtin_amdgcn_readfirstlane(min((int)stile * A.source_tile + A.source_tile, I.ns));
   return true;
 }
 
@@ -439,15 +446,17 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
 }
 
 // Shadow factor of the primary leg: obstructions between the sample and
-// the source (render.py:118-128), from the facet's candidate list when the
-// launch culls.  OBST: 0 = the scene has no obstructions, 1 = it has, 2 =
-// test at run time.
+// the source (render.py:118-128), tested from the facet's candidate mask
+// (every obstruction when the launch does not cull; the host builds the
+// masks before any trace of a scene with obstructions).  OBST: 0 = the scene
+// has no obstructions, 1 = it has, 2 = test at run time.
 template <int OBST>
 IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool active) {
   if (!(OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))) return 1.f;
-  return A.cull ? shadow_factor_culled(A.ob, as_const(A.cull) + (size_t)s.f * A.cull_words,
-                                       A.cull_words, s.p, -d, active)
-                : shadow_factor(A.ob, s.p, -d, active);
+  // the facet is wave-uniform in every kernel; saying so keeps the mask and
+  // obstruction reads scalar where the compiler cannot prove it
+  const int off = __builtin_amdgcn_readfirstlane(s.f * A.cull_words);
+  return shadow_factor_culled(A.ob, as_const(A.cull) + off, A.cull_words, s.p, -d, active);
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
