@@ -305,7 +305,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, scale, gi, pw, nw);
     Sample smp = make_sample<KIND>(A.S, pw, nw, wgt, pos);
-    smp.f = f;
+    set_facet(smp, facet_cull(A, f));
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
     const int s0 = w.s0, s1 = w.s1;

@@ -399,19 +399,21 @@ IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool activ
 // obstruction 32 w + j in the order cylinders, boxes, spheres, oriented
 // boxes, triangles.  The list is a superset of the obstructions the rays can
 // hit, so the result is shadow_factor's.
+// The facet's candidate mask is `mask` (words words); the caller has read
+// its first word `first` and its "any bit set" flag `any` once per facet, so
+// a ray of a facet without candidates (most facets of a dish) leaves at
+// once, before the per-ray set-up that the compiler hoists out of the tests,
+// and without a scalar-memory wait (which would also wait for the caller's
+// prefetches).
 IACT_DEV float shadow_factor_culled(const iact_obstructions& ob, cptr<uint32_t> mask, int words,
-                                    f3 o, f3 d, bool active) {
+                                    uint32_t first, int any, f3 o, f3 d, bool active) {
   const float big = 1e10f;
   const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
   const int c3 = c2 + ob.n_oriented_boxes;
-  // most facets of a dish have no candidate at all: leave before the
-  // per-ray set-up that the compiler hoists out of the tests
-  uint32_t any = 0u;
-  for (int w = 0; w < words; ++w) any |= mask[w];
-  if (any == 0u) return 1.f;
+  if (!any) return 1.f;
   bool blocked = !active;
   for (int w = 0; w < words; ++w) {
-    uint32_t bits = mask[w];
+    uint32_t bits = w == 0 ? first : mask[w];
     while (bits != 0u) {
       if (__all(blocked)) return (blocked && active) ? 0.f : 1.f;
       const int j = 32 * w + __builtin_ctz(bits);

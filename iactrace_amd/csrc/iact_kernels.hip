@@ -408,7 +408,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
     Sample smp = make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos);
-    smp.f = f;
+    set_facet(smp, facet_cull(A, f));
     const int s0 = w.s0, s1 = w.s1;
     float* const gout = A.out + (size_t)w.inst * P;
     const double* src = A.sources + 3 * w.sbase;
@@ -552,20 +552,24 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const float4* recs = (const float4*)A.recs;
     int run_pix = -1;
     float run_v = 0.f;
-    // facet of the current samples and the lane source's offset from it
+    // facet of the current samples, its obstruction candidates and the lane
+    // source's offset from it
     int fend = -1;
     int fcur = 0;
+    FacetCull fc{0, 0u, 0};
     f3 e = mk3(0.f, 0.f, 0.f);
     auto trace_one = [&](const RecRaw& rec, int j) {
       if (j >= fend) {
         fcur = (int)fdiv((uint32_t)j, A.d_m);
         fend = (fcur + 1) * A.M;
+        if (OBST) fc = facet_cull(A, fcur);
         const cptr<float> fp = as_const(A.positions) + 3 * fcur;  // scalar loads
         const f3 pos = mk3(fp[0], fp[1], fp[2]);
         e = source_offset<SRC>(sx, sy, sz, pos);
       }
       Sample smp = unpack_record(rec);
-      smp.f = fcur;
+      fc.f = fcur;
+      set_facet(smp, fc);
       float gu, gv, v, x, y;
       const bool ok =
           trace_ray<SRC, KIND, STAGES, AXIS, OBST>(A, smp, true, e, value, gu, gv, v, x, y);
@@ -651,7 +655,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
         fe = f;
       }
       Sample smp = make_sample<KIND>(A.S, p, n, A.world[6 * total + gi], pos);
-      smp.f = f;
+      set_facet(smp, facet_cull(A, f));
       float gu, gv, v, x, y;
       const bool ok = trace_ray<SRC, KIND, true, false>(A, smp, active, e, val, gu, gv, v, x, y);
       if (!SPLAT) {

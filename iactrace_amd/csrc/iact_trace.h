@@ -223,7 +223,34 @@ struct Sample {
   PlaneK K;        // sensor-plane constants of the primary origin
   GridK G;         // the same in sensor-grid coordinates
   int f;           // facet (selects its obstruction candidate list)
+  uint32_t cand0;  // the facet's first candidate-mask word
+  int cand_any;    // the facet has an obstruction candidate
 };
+
+// A facet's obstruction candidates as the trace needs them per ray (read
+// once per facet with scalar loads; the facet is wave-uniform).
+struct FacetCull {
+  int f;
+  uint32_t first;
+  int any;
+};
+IACT_DEV FacetCull facet_cull(const TraceArgs& A, int f) {
+  FacetCull c{f, 0u, 0};
+  if (A.cull) {
+    const cptr<uint32_t> m =
+        as_const(A.cull) + __builtin_amdgcn_readfirstlane(f * A.cull_words);
+    uint32_t any = m[0];
+    c.first = any;
+    for (int w = 1; w < A.cull_words; ++w) any |= m[w];
+    c.any = any != 0u;
+  }
+  return c;
+}
+IACT_DEV void set_facet(Sample& s, const FacetCull& c) {
+  s.f = c.f;
+  s.cand0 = c.first;
+  s.cand_any = c.any;
+}
 
 template <int KIND>
 IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_pos) {
@@ -235,6 +262,8 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_po
   s.K = plane_consts(S, p);
   s.G = grid_consts<KIND>(S, s.K);
   s.f = 0;
+  s.cand0 = 0u;
+  s.cand_any = 0;
   return s;
 }
 
@@ -312,6 +341,8 @@ IACT_DEV Sample unpack_record(const RecRaw& q) {
   s.p = mk3(q.c.z, q.c.w, q.d);
   s.K = PlaneK{0.f, 0.f, 0.f};  // world hit coordinates are not traced from records
   s.f = 0;
+  s.cand0 = 0u;
+  s.cand_any = 0;
   return s;
 }
 
@@ -456,7 +487,8 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
   // the facet is wave-uniform in every kernel; saying so keeps the mask and
   // obstruction reads scalar where the compiler cannot prove it
   const int off = __builtin_amdgcn_readfirstlane(s.f * A.cull_words);
-  return shadow_factor_culled(A.ob, as_const(A.cull) + off, A.cull_words, s.p, -d, active);
+  return shadow_factor_culled(A.ob, as_const(A.cull) + off, A.cull_words, s.cand0, s.cand_any,
+                              s.p, -d, active);
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
