@@ -393,9 +393,11 @@ static size_t base_workspace_bytes(const iact_scene& sc, int mode, const LaunchS
   return 0;
 }
 
-// Culling tables follow the mode's own workspace: 8 floats of facet cone and
-// cull_words mask words per facet.
-static size_t cull_offset(size_t base) { return (base + 255) & ~(size_t)255; }
+// Culling tables follow the mode's own workspace (256-B aligned pieces):
+// 8 floats of facet cone and cull_words mask words per facet; with chunk
+// culling the masks per (source chunk, facet); with source sorting the sort
+// scratch, the permutation and the sorted sources and values.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all) {
   const int cm = L ? L->culling : 0;
@@ -406,28 +408,79 @@ int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& 
   return IACT_OK;
 }
 
-size_t cull_bytes(const TraceArgs& A) {
-  return A.cull_words > 0 ? (size_t)A.nf * (8 + A.cull_words) * sizeof(float) : 0;
+CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& c) {
+  CullLayout L;
+  memset(&L, 0, sizeof(L));
+  if (words == 0) {
+    L.end = base;
+    return L;
+  }
+  size_t o = align256(base);
+  L.cones = o;
+  o += (size_t)nf * 8 * sizeof(float);
+  L.fmasks = o;
+  o = align256(o + (size_t)nf * words * sizeof(uint32_t));
+  L.chunks = !all && c.n_src >= kChunkCullMinSources && c.n_src < (1ll << 31);
+  if (L.chunks) {
+    L.cmasks = o;
+    o = align256(o + (size_t)((c.n_src + 63) / 64) * nf * words * sizeof(uint32_t));
+  }
+  L.sort = L.chunks && c.sort;
+  if (L.sort) {
+    L.sws = o;
+    o = align256(o + sort_workspace_bytes(c.n_src, c.n_inst));
+    L.perm = o;
+    o = align256(o + (size_t)c.n_src * sizeof(int32_t));
+    L.src = o;
+    o = align256(o + (size_t)c.n_src * 3 * sizeof(double));
+    L.val = o;
+    o = align256(o + (size_t)c.n_src * sizeof(float));
+  }
+  L.end = o;
+  return L;
 }
 
 size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
                            const TraceArgs& A) {
-  (void)ns;
   const size_t base = base_workspace_bytes(sc, mode, shp, A);
-  return A.cull_words > 0 ? cull_offset(base) + cull_bytes(A) : base;
+  const CullSpec c{ns, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det};
+  return cull_layout(A.nf, A.cull_words, A.cull_all, base, c).end;
 }
 
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
-                 long long n_cone_sources) {
-  if (A.cull_words == 0 || A.nf == 0 || A.ns == 0) return IACT_OK;
-  float* cones = (float*)((char*)workspace + cull_offset(base));
-  uint32_t* masks = (uint32_t*)(cones + 8 * (size_t)A.nf);
-  // the cones cover A.sources[0, n_cone_sources) (every instance of a batch)
+                 const CullSpec& c, const int32_t** perm) {
+  if (perm) *perm = nullptr;
+  A.cull_chunks = 0;
+  if (A.cull_words == 0 || A.nf == 0 || c.n_src == 0) return IACT_OK;
+  const CullLayout L = cull_layout(A.nf, A.cull_words, A.cull_all, base, c);
+  char* ws = (char*)workspace;
+  hipError_t e;
+  if (L.sort) {
+    int32_t* pm = (int32_t*)(ws + L.perm);
+    double* so = (double*)(ws + L.src);
+    float* vo = (float*)(ws + L.val);
+    e = launch_sort_sources(A.sources, A.values, c.n_src, c.offsets, c.n_inst, source_type,
+                            ws + L.sws, pm, so, vo, st);
+    if (e != hipSuccess) return hip_error(e, "source sort");
+    A.sources = so;
+    A.values = vo;
+    if (perm) *perm = pm;
+  }
+  float* cones = (float*)(ws + L.cones);
+  uint32_t* fmasks = (uint32_t*)(ws + L.fmasks);
+  // the cones cover every source of the trace (every instance of a batch)
   TraceArgs C = A;
-  if (n_cone_sources >= 0) C.ns = (int)n_cone_sources;
-  const hipError_t e = launch_cull(C, source_type, A.cull_all, cones, masks, st);
+  C.ns = (int)c.n_src;
+  e = launch_cull(C, source_type, A.cull_all, cones, fmasks, st);
   if (e != hipSuccess) return hip_error(e, "k_facet_cones/k_cull_masks");
-  A.cull = masks;
+  A.cull = fmasks;
+  if (L.chunks) {
+    uint32_t* cm = (uint32_t*)(ws + L.cmasks);
+    e = launch_chunk_masks(C, source_type, c.n_src, cones, fmasks, cm, st);
+    if (e != hipSuccess) return hip_error(e, "k_chunk_masks");
+    A.cull = cm;
+    A.cull_chunks = 1;
+  }
   return IACT_OK;
 }
 
@@ -589,7 +642,9 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
   hipError_t e = hipSuccess;
   const bool empty = (sc.facets.n_facets == 0 || n_sources == 0);
   if (!empty) {
-    rc = prepare_cull(A, workspace, base_workspace_bytes(sc, mode, shp, A), source_type, st);
+    const CullSpec c{n_sources, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det};
+    rc = prepare_cull(A, workspace, base_workspace_bytes(sc, mode, shp, A), source_type, st, c,
+                      nullptr);
     if (rc) return rc;
   }
   if (mode == IACT_MODE_IMAGE) {

@@ -22,13 +22,28 @@ int regrid(long long items, const iact_launch* L, const DevInfo& dev, LaunchShap
 // Obstruction culling: automatic from this many obstructions (a dish with a
 // camera box and four masts already traces 2.3x faster culled).  Every scene
 // with obstructions gets candidate masks (all bits set when not culling);
-// the tables of cull_bytes() follow a mode's own workspace of `base` bytes
-// and are filled by prepare_cull (which also points A.cull at the masks).
+// from kChunkCullMinSources sources the masks are per (source chunk of 64,
+// facet), and traces that may reorder their sources sort them into compact
+// direction cells first.  The tables follow a mode's own workspace of `base`
+// bytes (cull_layout) and are filled by prepare_cull, which points A.cull at
+// the masks, A.sources/A.values at the sorted copies and *perm at the
+// permutation (sorted -> caller index) when it sorts.
 constexpr int kAutoCullObstructions = 1;
+constexpr long long kChunkCullMinSources = 128;
+struct CullSpec {
+  long long n_src;         // sources of the trace (all instances)
+  int n_inst;              // instances (1 for a plain trace)
+  const int64_t* offsets;  // device (n_inst + 1) source prefix, or NULL
+  bool sort;               // the trace may reorder its sources
+};
+struct CullLayout {
+  size_t cones, fmasks, cmasks, sws, perm, src, val, end;  // byte offsets
+  bool chunks, sort;
+};
 int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all);
-size_t cull_bytes(const TraceArgs& A);
+CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& c);
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
-                 long long n_cone_sources = -1);
+                 const CullSpec& c, const int32_t** perm);
 int set_error(int code, const char* fmt, ...);
 int hip_error(hipError_t e, const char* where);
 int validate_scene(const iact_scene* sc);

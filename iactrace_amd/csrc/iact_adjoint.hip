@@ -305,18 +305,22 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, scale, gi, pw, nw);
     Sample smp = make_sample<KIND>(A.S, pw, nw, wgt, pos);
-    set_facet(smp, facet_cull(A, f));
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
     const int s0 = w.s0, s1 = w.s1;
     const double* src = A.sources + 3 * w.sbase;
     const cptr<float> vals = as_const(A.values + w.sbase);
     f3 e_lane = mk3(0.f, 0.f, 0.f);  // source offsets, 64 at a time (see k_trace)
+    long long chunk = -1;            // candidate mask per source chunk (see k_trace)
     for (int s = s0; s < s1; ++s) {
       const int k = (s - s0) & 63;
       if (k == 0) {
         const int sl = min(s + lane, s1 - 1);
         e_lane = source_offset<SRC>(src[3 * sl], src[3 * sl + 1], src[3 * sl + 2], pos);
+      }
+      if (((w.sbase + s) >> 6) != chunk) {
+        chunk = (w.sbase + s) >> 6;
+        set_facet(smp, facet_cull(A, f, chunk));
       }
       const f3 e = lane_value(e_lane, k);
       const float value = vals[s];
@@ -396,12 +400,14 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
         const float cz = want_src ? wave_sum(active ? gsz : 0.f) : 0.f;
         const float e = want_val ? wave_sum(gv * vunit) : 0.f;
         if (lane == 0) {
+          // sources the host sorted report to the caller's index
+          const long long so = B.src_perm ? (long long)as_const(B.src_perm)[s] : (long long)s;
           if (want_src) {
-            unsafeAtomicAdd(B.g_src + 3 * s, (double)a);
-            unsafeAtomicAdd(B.g_src + 3 * s + 1, (double)b);
-            unsafeAtomicAdd(B.g_src + 3 * s + 2, (double)cz);
+            unsafeAtomicAdd(B.g_src + 3 * so, (double)a);
+            unsafeAtomicAdd(B.g_src + 3 * so + 1, (double)b);
+            unsafeAtomicAdd(B.g_src + 3 * so + 2, (double)cz);
           }
-          if (want_val) unsafeAtomicAdd(B.g_val + s, (double)e);
+          if (want_val) unsafeAtomicAdd(B.g_val + so, (double)e);
         }
       }
     }
@@ -529,7 +535,8 @@ size_t iact_bwd_workspace_size(const iact_scene* scene, int32_t n_sources,
   fill_trace_args(A, *scene, nullptr, nullptr, n_sources);
   LaunchShape shp;
   if (shape_trace(*scene, n_sources, IACT_MODE_DEBUG, launch, A, shp, dev, -1)) return 0;
-  return cull_bytes(A);
+  const CullSpec c{n_sources, 1, nullptr, true};
+  return cull_layout(A.nf, A.cull_words, A.cull_all, 0, c).end;
 }
 
 int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const float* values,
@@ -577,7 +584,8 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   LaunchShape shp;
   rc = shape_trace(sc, n_sources, IACT_MODE_DEBUG, launch, B.T, shp, dev, -1);
   if (rc) return rc;
-  const size_t need = cull_bytes(B.T);
+  const CullSpec cspec{n_sources, 1, nullptr, true};
+  const size_t need = cull_layout(B.T.nf, B.T.cull_words, B.T.cull_all, 0, cspec).end;
   if (need > 0 && (!workspace || workspace_bytes < need))
     return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
                      workspace_bytes);
@@ -595,7 +603,7 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   B.offsets = sc.facets.offsets;
   rc = regrid(B.T.n_items, launch, dev, shp, B.T);
   if (rc) return rc;
-  rc = prepare_cull(B.T, workspace, 0, source_type, st);
+  rc = prepare_cull(B.T, workspace, 0, source_type, st, cspec, &B.src_perm);
   if (rc) return rc;
   e = launch_adjoint(B, shp, source_type, st);
   return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_adj");

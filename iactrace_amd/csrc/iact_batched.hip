@@ -171,8 +171,9 @@ static int batch_need(const iact_scene* scene, const iact_batch* batch, const ia
   int words = 0, all = 0;
   rc = cull_words_for(*scene, launch, words, all);
   if (rc) return rc;
-  const size_t c = words ? (size_t)scene->facets.n_facets * (8 + words) * sizeof(float) : 0;
-  need = batch_ws_bytes(batch->n_instances) + align256(c);
+  if (total >= (1ll << 31)) all = 1;  // as batch_shape
+  const CullSpec c{total, batch->n_instances, batch->source_offsets, true};
+  need = cull_layout(scene->facets.n_facets, words, all, batch_ws_bytes(batch->n_instances), c).end;
   return IACT_OK;
 }
 
@@ -200,7 +201,9 @@ static int prepare_batch(const iact_scene* scene, const iact_batch* batch, int s
   rc = batch_shape(scene, batch, source_type, launch, A, shp, L, dev, total);
   if (rc) return rc;
   if (tables && total > 0) {
-    rc = prepare_cull(A, workspace, batch_ws_bytes(batch->n_instances), source_type, st, total);
+    const CullSpec c{total, batch->n_instances, batch->source_offsets, true};
+    rc = prepare_cull(A, workspace, batch_ws_bytes(batch->n_instances), source_type, st, c,
+                      nullptr);
     if (rc) return rc;
   }
   const int wpb = shp.threads / 64;

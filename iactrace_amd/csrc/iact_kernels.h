@@ -33,6 +33,17 @@ hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_
 hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st);
 hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cones,
                        uint32_t* masks, hipStream_t st);
+hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_sources,
+                              const float* cones, const uint32_t* facet_masks, uint32_t* out,
+                              hipStream_t st);
+// Source ordering for chunk culling (iact_sort.hip): sorts the sources of
+// each instance (offsets: n_inst + 1 device prefix, or NULL for one) into
+// direction cells; writes perm (sorted -> original), sorted sources/values.
+size_t sort_workspace_bytes(long long n_sources, int n_inst);
+hipError_t launch_sort_sources(const double* src, const float* val, long long n_sources,
+                               const int64_t* offsets, int n_inst, int source_type,
+                               void* ws, int32_t* perm, double* src_out, float* val_out,
+                               hipStream_t st);
 hipError_t launch_reduce(const float* partials, int nblocks, int P, float* out, hipStream_t st);
 
 // Arguments of the adjoint kernel (iact_adjoint.hip).
@@ -46,6 +57,7 @@ struct AdjArgs {
   double* g_asph;
   const int32_t* surface_index;
   const float* offsets;
+  const int32_t* src_perm;  // sorted -> caller source index (NULL: not sorted)
 };
 hipError_t launch_adjoint(const AdjArgs& B, const LaunchShape& L, int source_type,
                           hipStream_t st);

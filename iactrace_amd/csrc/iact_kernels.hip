@@ -237,14 +237,53 @@ IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
   return !(s + rr < 0.f) && !(e > fmaf(tn, s + rr, rr));
 }
 
-// Candidate bits: one thread per (facet, mask word).  Long cylinders are
-// tested as a chain of bounding spheres of pieces at most 1 m long (at most
-// 64 pieces).
+// Can a ray of the bundle reach obstruction j (mask numbering: cylinders,
+// boxes, spheres, oriented boxes, triangles)?  Long cylinders are tested as
+// a chain of bounding spheres of pieces at most 1 m long (at most 64).
+IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
+                                  float tn) {
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
+  const int c3 = c2 + ob.n_oriented_boxes;
+  if (j < c0) {
+    const float* q = ob.cylinders + 8 * j;
+    const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
+    const float h = q[6], r = q[7];
+    const int n = (int)fminf(fmaxf(ceilf(h), 1.f), 64.f);
+    const float hp = h / (float)n;
+    const float R = sqrtf(0.25f * hp * hp + r * r);
+    for (int i = 0; i < n; ++i)
+      if (bundle_reaches(c, rho, a, tn, fma3(hp * ((float)i + 0.5f), ax, p1), R)) return true;
+    return false;
+  }
+  if (j < c1) {
+    const float* q = ob.boxes + 6 * (j - c0);
+    const f3 lo = mk3(q[0], q[1], q[2]), hi = mk3(q[3], q[4], q[5]);
+    const f3 dd = hi - lo;
+    return bundle_reaches(c, rho, a, tn, 0.5f * (lo + hi), 0.5f * sqrtf(dot(dd, dd)));
+  }
+  if (j < c2) {
+    const float* q = ob.spheres + 4 * (j - c1);
+    return bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), q[3]);
+  }
+  if (j < c3) {
+    const float* q = ob.oriented_boxes + 15 * (j - c2);
+    const f3 he = mk3(q[3], q[4], q[5]);
+    return bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), sqrtf(dot(he, he)));
+  }
+  const float* q = ob.triangles + 9 * (j - c3);
+  const f3 v0 = mk3(q[0], q[1], q[2]), v1 = mk3(q[3], q[4], q[5]);
+  const f3 v2 = mk3(q[6], q[7], q[8]);
+  const f3 ctr = (1.f / 3.f) * (v0 + v1 + v2);
+  const f3 e0 = v0 - ctr, e1 = v1 - ctr, e2 = v2 - ctr;
+  const float R = sqrtf(fmaxf(fmaxf(dot(e0, e0), dot(e1, e1)), dot(e2, e2)));
+  return bundle_reaches(c, rho, a, tn, ctr, R);
+}
+
+// Candidate bits: one thread per (facet, mask word).
 __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, int all) {
   const iact_obstructions& ob = A.ob;
   const int words = A.cull_words;
-  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
-  const int c3 = c2 + ob.n_oriented_boxes, c4 = c3 + ob.n_triangles;
+  const int n_ob = n_obstructions(ob);
   const long long total = (long long)A.nf * words;
   for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (long long)gridDim.x * blockDim.x) {
@@ -254,44 +293,86 @@ __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, i
     const float rho = all ? 0.f : cn[3], tn = all ? -1.f : cn[7];
     const f3 a = all ? mk3(0.f, 0.f, 1.f) : mk3(cn[4], cn[5], cn[6]);
     uint32_t bits = 0u;
-    for (int j = 32 * w; j < min(32 * w + 32, c4); ++j) {
-      bool hit;
-      if (all || tn < 0.f) {
-        hit = true;
-      } else if (j < c0) {
-        const float* q = ob.cylinders + 8 * j;
-        const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
-        const float h = q[6], r = q[7];
-        const int n = (int)fminf(fmaxf(ceilf(h), 1.f), 64.f);
-        const float hp = h / (float)n;
-        const float R = sqrtf(0.25f * hp * hp + r * r);
-        hit = false;
-        for (int i = 0; i < n && !hit; ++i)
-          hit = bundle_reaches(c, rho, a, tn, fma3(hp * ((float)i + 0.5f), ax, p1), R);
-      } else if (j < c1) {
-        const float* q = ob.boxes + 6 * (j - c0);
-        const f3 lo = mk3(q[0], q[1], q[2]), hi = mk3(q[3], q[4], q[5]);
-        const f3 dd = hi - lo;
-        hit = bundle_reaches(c, rho, a, tn, 0.5f * (lo + hi), 0.5f * sqrtf(dot(dd, dd)));
-      } else if (j < c2) {
-        const float* q = ob.spheres + 4 * (j - c1);
-        hit = bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), q[3]);
-      } else if (j < c3) {
-        const float* q = ob.oriented_boxes + 15 * (j - c2);
-        const f3 he = mk3(q[3], q[4], q[5]);
-        hit = bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), sqrtf(dot(he, he)));
-      } else {
-        const float* q = ob.triangles + 9 * (j - c3);
-        const f3 v0 = mk3(q[0], q[1], q[2]), v1 = mk3(q[3], q[4], q[5]);
-        const f3 v2 = mk3(q[6], q[7], q[8]);
-        const f3 ctr = (1.f / 3.f) * (v0 + v1 + v2);
-        const f3 e0 = v0 - ctr, e1 = v1 - ctr, e2 = v2 - ctr;
-        const float R = sqrtf(fmaxf(fmaxf(dot(e0, e0), dot(e1, e1)), dot(e2, e2)));
-        hit = bundle_reaches(c, rho, a, tn, ctr, R);
-      }
-      if (hit) bits |= 1u << (j - 32 * w);
-    }
+    for (int j = 32 * w; j < min(32 * w + 32, n_ob); ++j)
+      if (all || tn < 0.f || obstruction_reaches(ob, j, c, rho, a, tn)) bits |= 1u << (j - 32 * w);
     masks[id] = bits;
+  }
+}
+
+// Candidate bits per (source chunk, facet): the bundle of one facet's rays
+// towards the 64 sources of one chunk (consecutive sources, which the host
+// sorts into compact direction cells) is much narrower than the facet's
+// bundle towards all sources.  One thread per (chunk, facet); the facet's
+// own mask is the starting list, its cone gives the sample radius rho.
+// masks_out[(chunk * nf + f) * words + w].
+__global__ void k_chunk_masks(TraceArgs A, int source_type, long long n_sources,
+                              const float* cones, const uint32_t* facet_masks,
+                              uint32_t* masks_out) {
+  const iact_obstructions& ob = A.ob;
+  const int words = A.cull_words;
+  const long long n_chunks = (n_sources + 63) / 64;
+  const long long total = n_chunks * A.nf;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const long long ch = id / A.nf;
+    const int f = (int)(id - ch * A.nf);
+    const float* cn = cones + 8 * (size_t)f;
+    const f3 c = mk3(cn[0], cn[1], cn[2]);
+    const float rho = cn[3];
+    const long long s0 = ch * 64, s1 = min(s0 + 64, n_sources);
+    // axis: mean unit direction towards the chunk's sources; then the half
+    // angle with the parallax of point sources (as k_facet_cones)
+    f3 sum = mk3(0.f, 0.f, 0.f);
+    for (long long s = s0; s < s1; ++s) {
+      f3 u;
+      if (source_type == IACT_SOURCE_POINT) {
+        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
+        const double uz = A.sources[3 * s + 2] - (double)c.z;
+        const double il = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
+        u = mk3((float)(ux * il), (float)(uy * il), (float)(uz * il));
+      } else {
+        u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
+                -(float)A.sources[3 * s + 2]);
+        u = frsq(dot(u, u)) * u;
+      }
+      sum = sum + u;
+    }
+    const float la = sqrtf(dot(sum, sum));
+    const f3 a = la > 0.f ? (1.f / la) * sum : mk3(0.f, 0.f, 1.f);
+    float ang = 0.f;
+    for (long long s = s0; s < s1; ++s) {
+      float par = 0.f;
+      f3 u;
+      if (source_type == IACT_SOURCE_POINT) {
+        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
+        const double uz = A.sources[3 * s + 2] - (double)c.z;
+        const double l = sqrt(ux * ux + uy * uy + uz * uz);
+        u = mk3((float)(ux / l), (float)(uy / l), (float)(uz / l));
+        par = l > (double)rho ? asinf((float)((double)rho / l)) : 3.1415927f;
+      } else {
+        u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
+                -(float)A.sources[3 * s + 2]);
+        u = frsq(dot(u, u)) * u;
+      }
+      ang = fmaxf(ang, acosf(fminf(fmaxf(dot(u, a), -1.f), 1.f)) + par);
+    }
+    const float th = ang + 2e-4f;
+    const float tn = th < 1.4f ? tanf(th) : -1.f;
+    const uint32_t* fm = facet_masks + (size_t)f * words;
+    uint32_t* out = masks_out + (size_t)id * words;
+    for (int w = 0; w < words; ++w) {
+      uint32_t bits = fm[w], keep = 0u;
+      if (tn < 0.f) {
+        keep = bits;
+      } else {
+        while (bits) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          if (obstruction_reaches(ob, 32 * w + b, c, rho, a, tn)) keep |= 1u << b;
+        }
+      }
+      out[w] = keep;
+    }
   }
 }
 
@@ -408,20 +489,25 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     f3 pw, nw;
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
     Sample smp = make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos);
-    set_facet(smp, facet_cull(A, f));
     const int s0 = w.s0, s1 = w.s1;
     float* const gout = A.out + (size_t)w.inst * P;
     const double* src = A.sources + 3 * w.sbase;
     const cptr<float> vals = as_const(A.values + w.sbase);
     // the facet's source offsets, 64 sources at a time (lane k holds source
-    // s + k), read by the wave one by one
+    // s + k), read by the wave one by one; the candidate mask of the facet
+    // and the current source chunk
     f3 e_lane = mk3(0.f, 0.f, 0.f);
+    long long chunk = -1;
     float sv = vals[s0];
     for (int s = s0; s < s1; ++s) {
       const int k = (s - s0) & 63;
       if (k == 0) {
         const int sl = min(s + lane, s1 - 1);
         e_lane = source_offset<SRC>(src[3 * sl], src[3 * sl + 1], src[3 * sl + 2], pos);
+      }
+      if (((w.sbase + s) >> 6) != chunk) {
+        chunk = (w.sbase + s) >> 6;
+        set_facet(smp, facet_cull(A, f, chunk));
       }
       const f3 e = lane_value(e_lane, k);
       // prefetch the next value while this source is traced
@@ -562,7 +648,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       if (j >= fend) {
         fcur = (int)fdiv((uint32_t)j, A.d_m);
         fend = (fcur + 1) * A.M;
-        if (OBST) fc = facet_cull(A, fcur);
+        if (OBST) fc = facet_cull(A, fcur, c);
         const cptr<float> fp = as_const(A.positions) + 3 * fcur;  // scalar loads
         const f3 pos = mk3(fp[0], fp[1], fp[2]);
         e = source_offset<SRC>(sx, sy, sz, pos);
@@ -655,7 +741,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
         fe = f;
       }
       Sample smp = make_sample<KIND>(A.S, p, n, A.world[6 * total + gi], pos);
-      set_facet(smp, facet_cull(A, f));
+      set_facet(smp, facet_cull(A, f, s >> 6));
       float gu, gv, v, x, y;
       const bool ok = trace_ray<SRC, KIND, true, false>(A, smp, active, e, val, gu, gv, v, x, y);
       if (!SPLAT) {
@@ -805,6 +891,15 @@ hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cone
   const long long total = (long long)A.nf * A.cull_words;
   hipLaunchKernelGGL(k_cull_masks, dim3(grid_for(total, 256, 4096)), dim3(256), 0, st, A, cones,
                      masks, all);
+  return hipGetLastError();
+}
+
+hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_sources,
+                              const float* cones, const uint32_t* facet_masks, uint32_t* out,
+                              hipStream_t st) {
+  const long long total = (n_sources + 63) / 64 * A.nf;
+  hipLaunchKernelGGL(k_chunk_masks, dim3(grid_for(total, 256, 16384)), dim3(256), 0, st, A,
+                     source_type, n_sources, cones, facet_masks, out);
   return hipGetLastError();
 }
 

@@ -107,6 +107,7 @@ struct TraceArgs {
   const uint32_t* cull;
   int cull_words;
   int cull_all;                  // masks with every obstruction (culling off)
+  int cull_chunks;               // masks per (source chunk of 64, facet)
 };
 
 // World-space sample record of the source-major trace (16 words, read with
@@ -225,20 +226,31 @@ struct Sample {
   int f;           // facet (selects its obstruction candidate list)
   uint32_t cand0;  // the facet's first candidate-mask word
   int cand_any;    // the facet has an obstruction candidate
+  size_t cand_row; // offset of the facet's candidate mask
 };
 
+// Offset of mask row `row` (facet, or chunk * nf + facet), wave-uniform.
+IACT_DEV size_t mask_offset(const TraceArgs& A, long long row) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)row);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)row >> 32));
+  return (size_t)(((uint64_t)hi << 32) | lo) * (size_t)A.cull_words;
+}
+
 // A facet's obstruction candidates as the trace needs them per ray (read
-// once per facet with scalar loads; the facet is wave-uniform).
+// once per facet, or per facet and source chunk, with scalar loads; both
+// are wave-uniform).
 struct FacetCull {
   int f;
   uint32_t first;
   int any;
+  size_t row;
 };
-IACT_DEV FacetCull facet_cull(const TraceArgs& A, int f) {
-  FacetCull c{f, 0u, 0};
+IACT_DEV FacetCull facet_cull(const TraceArgs& A, int f, long long chunk) {
+  FacetCull c{f, 0u, 0, 0};
   if (A.cull) {
-    const cptr<uint32_t> m =
-        as_const(A.cull) + __builtin_amdgcn_readfirstlane(f * A.cull_words);
+    const long long row = A.cull_chunks ? chunk * A.nf + f : f;
+    c.row = mask_offset(A, row);
+    const cptr<uint32_t> m = as_const(A.cull) + c.row;
     uint32_t any = m[0];
     c.first = any;
     for (int w = 1; w < A.cull_words; ++w) any |= m[w];
@@ -250,6 +262,7 @@ IACT_DEV void set_facet(Sample& s, const FacetCull& c) {
   s.f = c.f;
   s.cand0 = c.first;
   s.cand_any = c.any;
+  s.cand_row = c.row;
 }
 
 template <int KIND>
@@ -264,6 +277,7 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_po
   s.f = 0;
   s.cand0 = 0u;
   s.cand_any = 0;
+  s.cand_row = 0;
   return s;
 }
 
@@ -343,6 +357,7 @@ IACT_DEV Sample unpack_record(const RecRaw& q) {
   s.f = 0;
   s.cand0 = 0u;
   s.cand_any = 0;
+  s.cand_row = 0;
   return s;
 }
 
@@ -484,11 +499,8 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
 template <int OBST>
 IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool active) {
   if (!(OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))) return 1.f;
-  // the facet is wave-uniform in every kernel; saying so keeps the mask and
-  // obstruction reads scalar where the compiler cannot prove it
-  const int off = __builtin_amdgcn_readfirstlane(s.f * A.cull_words);
-  return shadow_factor_culled(A.ob, as_const(A.cull) + off, A.cull_words, s.cand0, s.cand_any,
-                              s.p, -d, active);
+  return shadow_factor_culled(A.ob, as_const(A.cull) + s.cand_row, A.cull_words, s.cand0,
+                              s.cand_any, s.p, -d, active);
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).

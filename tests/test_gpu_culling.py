@@ -40,16 +40,18 @@ def _debug_pair(tel, src, vals, stype, label):
     return vb
 
 
-@pytest.mark.parametrize("stype", ["point", "parallel"])
-def test_truss_debug_bitwise(stype):
-    wl = configs.workload("C3T", n_sources=12, n_samples=64)
+@pytest.mark.parametrize("stype,ns", [("point", 12), ("parallel", 12), ("point", 160),
+                                      ("parallel", 192)])
+def test_truss_debug_bitwise(stype, ns):
+    """(from 128 sources the masks are per source chunk)"""
+    wl = configs.workload("C3T", n_sources=ns, n_samples=64 if ns < 100 else 16)
     tel = wl.telescope(DEV, key=5)
     if stype == "point":
         src, vals = wl.sources, wl.values
     else:
-        src = configs.parallel_grid(4, 3, 0.8)
+        src = configs.parallel_grid(4 if ns < 100 else 16, 3 if ns < 100 else 12, 0.3)
         vals = np.linspace(0.5, 2.0, len(src)).astype(np.float32)
-    vb = _debug_pair(tel, src, vals, stype, f"truss-{stype}")
+    vb = _debug_pair(tel, src, vals, stype, f"truss-{stype}-{len(src)}")
     # the truss does cast shadows on this workload
     assert (vb == 0).mean() > 0.01
 
@@ -140,6 +142,34 @@ def test_truss_images_response_and_gradients():
     err = (grads[0] - grads[1]).abs().max().item() / grads[1].abs().max().item()
     print(f"[truss-grad] max rel err {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+def test_chunk_culling_sorted_sources_and_source_gradients():
+    """From 128 sources the masks are per (source chunk, facet) and image and
+    gradient traces sort the sources into direction cells: images, facet and
+    per-source gradients (written back through the permutation) must match
+    the unculled traces."""
+    wl = configs.workload("C3T", n_sources=300, n_samples=32)
+    tel = wl.telescope(DEV, key=8)
+    W = torch.linspace(0.5, 1.5, tel.sensors[0].n_pixels, device=DEV)
+    out = []
+    for L in (ON, OFF):
+        g = tel.mirror_groups[0]
+        rot = g.rotations.detach().clone().requires_grad_(True)
+        t = ia.Telescope([g.replace(rotations=rot)], tel.obstruction_groups, tel.sensors)
+        s = torch.as_tensor(wl.sources, device=DEV).clone().requires_grad_(True)
+        v = torch.as_tensor(wl.values, device=DEV).clone().requires_grad_(True)
+        img = ia.render(t, s, v, "point", launch=L)
+        (img.reshape(-1) * W).sum().backward()
+        out.append((img.detach(), rot.grad.double(), s.grad.double(), v.grad.double()))
+    names = ["image", "rotations", "sources", "values"]
+    for k in range(4):
+        a, b = out[0][k], out[1][k]
+        err = (a - b).abs().max().item() / max(b.abs().max().item(), 1e-300)
+        print(f"[chunk-culling] {names[k]}: max rel err {err:.2e}", flush=True)
+        assert err < 1e-5, names[k]
+    # the truss shadows part of the image
+    assert out[1][0].abs().sum().item() > 0
 
 
 def test_culling_mode_validation():
