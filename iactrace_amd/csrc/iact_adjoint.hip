@@ -295,16 +295,15 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     const size_t gi = (size_t)f * A.M + (active ? m : A.M - 1);
     const float* rot = A.rotations + w.inst * A.rot_stride;
     double* const g_rot = B.g_rot ? B.g_rot + (size_t)w.inst * 3 * A.nf : nullptr;
-    const m3 R = facet_rotation(rot, f);
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
-    const f3 pl = mk3(A.points[3 * gi], A.points[3 * gi + 1], A.points[3 * gi + 2]);
-    const f3 nl = mk3(A.normals[3 * gi], A.normals[3 * gi + 1], A.normals[3 * gi + 2]);
-    const f3 dl = mk3(A.delta[3 * gi], A.delta[3 * gi + 1], A.delta[3 * gi + 2]);
-    const float scale = A.pscale[f];
-    const float wgt = A.weights[gi];
-    f3 pw, nw;
-    world_sample(A.points, A.normals, A.delta, R, pos, scale, gi, pw, nw);
-    Sample smp = make_sample<KIND>(A.S, pw, nw, wgt, pos);
+    f3 nw;
+    Sample smp;
+    {
+      const m3 R = facet_rotation(rot, f);
+      f3 pw;
+      world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
+      smp = make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos);
+    }
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
     const int s0 = w.s0, s1 = w.s1;
@@ -416,6 +415,16 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       An = mk3(0.f, 0.f, 0.f);
       Anrw = 0.f;
     }
+    // the sample's facet-frame data and pose are read again rather than kept
+    // in registers through the source loop (the barrier stops the compiler
+    // from reusing the first reads)
+    asm volatile("" ::: "memory");
+    const m3 R = facet_rotation(rot, f);
+    const f3 pl = mk3(A.points[3 * gi], A.points[3 * gi + 1], A.points[3 * gi + 2]);
+    const f3 nl = mk3(A.normals[3 * gi], A.normals[3 * gi + 1], A.normals[3 * gi + 2]);
+    const f3 dl = mk3(A.delta[3 * gi], A.delta[3 * gi + 1], A.delta[3 * gi + 2]);
+    const float scale = A.pscale[f];
+    const float wgt = A.weights[gi];
     // ---- per-sample adjoint through the world transform (mirrors.py:64-79)
     // n = pert / |pert|, pert = R (nl + s dl);  p = R pl + pos
     const f3 nls = fma3(scale, dl, nl);
