@@ -570,18 +570,67 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   }
 }
 
-// World-space records of every (facet, sample) for the source-major trace.
+// World-space records of every (facet, sample) for the source-major trace,
+// one block per facet.  Within a facet the records are written in the order
+// of a 16 x 16 Morton grid over the samples' aperture positions (a counting
+// sort in LDS): consecutive samples then lie close together on the facet,
+// their rays towards one source land on the same pixel for longer, and the
+// trace's per-lane runs of equal pixels are added to the image less often.
+// The order inside a cell is the order of the LDS atomics (an image sum does
+// not depend on it).
+IACT_DEV uint32_t spread4(uint32_t v) {  // 4 bits -> even positions
+  v &= 0xfu;
+  v = (v | (v << 2)) & 0x33u;
+  v = (v | (v << 1)) & 0x55u;
+  return v;
+}
+
 template <int KIND>
-__global__ void k_sample_records(TraceArgs A, float* recs) {
-  const long long total = (long long)A.nf * A.M;
-  for (long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x; gi < total;
-       gi += (long long)gridDim.x * blockDim.x) {
-    const int f = (int)(gi / A.M);
+__global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs) {
+  __shared__ uint32_t cnt[256];
+  __shared__ float ext[256];
+  const int t = threadIdx.x;
+  for (int f = blockIdx.x; f < A.nf; f += gridDim.x) {
+    const size_t g0 = (size_t)f * A.M;
+    // extent of the facet's aperture positions
+    float e = 0.f;
+    for (int m = t; m < A.M; m += blockDim.x)
+      e = fmaxf(e, fmaxf(fabsf(A.points[3 * (g0 + m)]), fabsf(A.points[3 * (g0 + m) + 1])));
+    ext[t] = e;
+    cnt[t] = 0u;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+      if (t < h) ext[t] = fmaxf(ext[t], ext[t + h]);
+      __syncthreads();
+    }
+    const float sc = 8.f / fmaxf(ext[0], 1e-30f);  // 16 cells over [-ext, ext]
+    auto key = [&](int m) {
+      const float x = A.points[3 * (g0 + m)], y = A.points[3 * (g0 + m) + 1];
+      const int cx = (int)fminf(fmaxf(floorf(fmaf(x, sc, 8.f)), 0.f), 15.f);
+      const int cy = (int)fminf(fmaxf(floorf(fmaf(y, sc, 8.f)), 0.f), 15.f);
+      return spread4((uint32_t)cx) | (spread4((uint32_t)cy) << 1);
+    };
+    for (int m = t; m < A.M; m += blockDim.x) atomicAdd(cnt + key(m), 1u);
+    __syncthreads();
+    if (t == 0) {  // exclusive scan of 256 counts
+      uint32_t run = 0u;
+      for (int i = 0; i < 256; ++i) {
+        const uint32_t c = cnt[i];
+        cnt[i] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
     const m3 R = facet_rotation(A.rotations, f);
     const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
-    f3 pw, nw;
-    world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], (size_t)gi, pw, nw);
-    store_record(recs + IACT_REC_WORDS * gi, make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos));
+    for (int m = t; m < A.M; m += blockDim.x) {
+      const uint32_t slot = atomicAdd(cnt + key(m), 1u);
+      f3 pw, nw;
+      world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], g0 + m, pw, nw);
+      store_record(recs + IACT_REC_WORDS * (g0 + slot),
+                   make_sample<KIND>(A.S, pw, nw, A.weights[g0 + m], pos));
+    }
+    __syncthreads();
   }
 }
 
@@ -906,10 +955,10 @@ hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_s
 hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st) {
   const long long total = (long long)A.nf * A.M;
   if (A.S.kind == IACT_SENSOR_SQUARE)
-    hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_SQUARE>, dim3(grid_for(total, 256, 16384)),
+    hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_SQUARE>, dim3(grid_for(A.nf, 1, 16384)),
                        dim3(256), 0, st, A, recs);
   else
-    hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_HEX>, dim3(grid_for(total, 256, 16384)),
+    hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_HEX>, dim3(grid_for(A.nf, 1, 16384)),
                        dim3(256), 0, st, A, recs);
   return hipGetLastError();
 }
