@@ -144,7 +144,8 @@ def test_truss_images_response_and_gradients():
     assert err < 1e-5
 
 
-def test_chunk_culling_sorted_sources_and_source_gradients():
+@pytest.mark.parametrize("stype", ["point", "parallel"])
+def test_chunk_culling_sorted_sources_and_source_gradients(stype):
     """From 128 sources the masks are per (source chunk, facet) and image and
     gradient traces sort the sources into direction cells: images, facet and
     per-source gradients (written back through the permutation) must match
@@ -152,21 +153,29 @@ def test_chunk_culling_sorted_sources_and_source_gradients():
     wl = configs.workload("C3T", n_sources=300, n_samples=32)
     tel = wl.telescope(DEV, key=8)
     W = torch.linspace(0.5, 1.5, tel.sensors[0].n_pixels, device=DEV)
+    if stype == "point":
+        src0, val0 = wl.sources, wl.values
+    else:
+        rng = np.random.default_rng(9)
+        src0 = configs.parallel_grid(20, 15, 0.15)[rng.permutation(300)]
+        val0 = rng.uniform(0.5, 2.0, 300).astype(np.float32)
     out = []
     for L in (ON, OFF):
         g = tel.mirror_groups[0]
         rot = g.rotations.detach().clone().requires_grad_(True)
         t = ia.Telescope([g.replace(rotations=rot)], tel.obstruction_groups, tel.sensors)
-        s = torch.as_tensor(wl.sources, device=DEV).clone().requires_grad_(True)
-        v = torch.as_tensor(wl.values, device=DEV).clone().requires_grad_(True)
-        img = ia.render(t, s, v, "point", launch=L)
+        s = torch.as_tensor(src0, device=DEV).clone().requires_grad_(True)
+        v = torch.as_tensor(val0, device=DEV).clone().requires_grad_(True)
+        img = ia.render(t, s, v, stype, launch=L)
         (img.reshape(-1) * W).sum().backward()
         out.append((img.detach(), rot.grad.double(), s.grad.double(), v.grad.double()))
     names = ["image", "rotations", "sources", "values"]
     for k in range(4):
         a, b = out[0][k], out[1][k]
-        err = (a - b).abs().max().item() / max(b.abs().max().item(), 1e-300)
-        print(f"[chunk-culling] {names[k]}: max rel err {err:.2e}", flush=True)
+        scale = b.abs().max().item()
+        err = (a - b).abs().max().item() / max(scale, 1e-300)
+        print(f"[chunk-culling] {names[k]}: max rel err {err:.2e} (scale {scale:.3g})", flush=True)
+        assert scale > 0, names[k]
         assert err < 1e-5, names[k]
     # the truss shadows part of the image
     assert out[1][0].abs().sum().item() > 0
@@ -192,7 +201,8 @@ def test_batched_culling_matches_unculled():
     rot[:, :, :2] += torch.as_tensor(rng.normal(0, 0.05, (n_inst, base.shape[0], 2)),
                                      dtype=torch.float32, device=DEV)
     src, vals = [], []
-    for i, n in enumerate((40, 0, 70)):
+    # 270 sources in all: per-chunk masks over sources sorted per instance
+    for i, n in enumerate((100, 0, 170)):
         a, b = configs.point_source_field(n, 2.0, seed=80 + i)
         src.append(a)
         vals.append(b)
