@@ -126,6 +126,44 @@ def c3t_culling(ia, configs, steps, warmup):
     return out
 
 
+def c4_response(ia, configs, steps, warmup):
+    """C4: two-mirror Cassegrain (aspheric primary, K = -1.05, four even
+    terms; hyperbolic secondary), square 64x64 sensor: the 8192 x 4096
+    response matrix of a 128 x 64 grid of parallel sources 0.01 deg apart
+    (2048 samples), and forward+backward of a weighted image wrt both
+    mirrors' aspheric coefficients (the adjoint through the secondary)."""
+    import torch
+    dev = torch.device("cuda:0")
+    wl = configs.workload("C4")
+    tel = wl.telescope(dev, key=0)
+    src = torch.as_tensor(wl.sources, dtype=torch.float64, device=dev)
+    vals = torch.as_tensor(wl.values, dtype=torch.float32, device=dev)
+    g0, g1 = tel.mirror_groups
+    a0 = g0.aspheric.detach().clone().requires_grad_(True)
+    a1 = torch.zeros(2).requires_grad_(True)  # secondary: two even terms, 0 here (host: launch constants)
+    t2 = ia.Telescope([g0.replace(aspheric=a0), g1.replace(aspheric=a1)], [], tel.sensors)
+    W = torch.linspace(0.5, 1.5, tel.sensors[0].n_pixels, device=dev)
+
+    def resp():
+        return ia.render_response_matrix(tel, src, vals, wl.source_type)
+
+    def fwd_bwd():
+        a0.grad = None
+        a1.grad = None
+        (ia.render(t2, src, vals, wl.source_type).reshape(-1) * W).sum().backward()
+
+    wr, _ = time_steps(resp, steps, warmup)
+    wb, _ = time_steps(fwd_bwd, steps, warmup)
+    return {"workload": f"C4: Cassegrain, {wl.n_facets} primary x {wl.n_samples} samples x "
+                        f"{len(wl.sources)} parallel sources (128 x 64 grid, 0.01 deg), "
+                        "square 64 x 64 sensor",
+            "rays_per_step": wl.n_rays, "steps": steps,
+            "response_matrix_rays_per_s": wl.n_rays * steps / wr,
+            "response_matrix_ms": 1e3 * wr / steps,
+            "fwd_bwd_aspheric_rays_per_s": wl.n_rays * steps / wb,
+            "fwd_bwd_aspheric_ms": 1e3 * wb / steps}
+
+
 def cpu_baseline(wl, tel, target_s=15.0):
     """Time the C/OpenMP port on a source subsample of the same workload."""
     import numpy as np
@@ -295,6 +333,7 @@ def main():
     if not args.no_extras and args.config == "C2":
         out["extras"] = {"C3_fwd_bwd": c3_fwd_bwd(ia, configs, 3, 1),
                          "C3T_culling": c3t_culling(ia, configs, 2, 1),
+                         "C4_response": c4_response(ia, configs, 5, 1),
                          "C5_batched": c5_batched(ia, configs, args.c5_instances)}
     if not args.no_cpu:
         cb, cimg, n = cpu_baseline(wl, tel)
