@@ -171,6 +171,8 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
   S.wf = (float)s.width;
   S.hf = (float)s.height;
   S.edge_width = s.edge_width;
+  // an integer flag: the per-ray edge branch is then a scalar compare
+  S.edge = s.kind == IACT_SENSOR_SQUARE ? (s.edge_width > 0.f) : (s.edge_threshold < 1.f);
   S.size = s.hex_size;
   S.inv_size = s.hex_size > 0.f ? 1.f / s.hex_size : 0.f;
   S.size_s3 = (float)((double)s.hex_size * 1.7320508075688772);
@@ -190,18 +192,21 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
     const double ox = s.grid_offset[0], oy = s.grid_offset[1];
     S.aqx = (float)aqx;
     S.aqy = (float)aqy;
-    S.aq0 = (float)(-(aqx * ox + aqy * oy));
+    // axial coordinates on the device are relative to the lookup table's
+    // origin (q - q_min, r - r_min): rounding commutes with the integer
+    // shift, and the table index needs no subtraction per ray
+    S.aq0 = (float)(-(aqx * ox + aqy * oy) - (double)s.q_min);
     S.arx = (float)arx;
     S.ary = (float)ary;
-    S.ar0 = (float)(-(arx * ox + ary * oy));
+    S.ar0 = (float)(-(arx * ox + ary * oy) - (double)s.r_min);
   }
   S.edge_thr = s.edge_threshold;
   S.q_min = s.q_min;
   S.r_min = s.r_min;
   S.tq = s.table_q;
   S.tr = s.table_r;
-  S.qmin_f = (float)s.q_min;
-  S.rmin_f = (float)s.r_min;
+  S.qmin_f = 0.f;
+  S.rmin_f = 0.f;
   S.tq_f = (float)s.table_q;
   S.tr_f = (float)s.table_r;
   S.table = s.lookup_table;
@@ -342,7 +347,11 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
     const long long total = (long long)A.nf * A.M;
     long long sg = (L && L->sample_group) ? L->sample_group : 64;
     if (sg < 1) return set_error(IACT_ERR_INVALID, "sample_group %lld", sg);
-    const long long n_sc = ns > 0 ? (ns + 63) / 64 : 1;
+    // two sources per lane from 128 sources: a sample's record, loop control
+    // and facet bookkeeping then serve two rays per lane
+    A.sm_spl = ns >= 128 ? 2 : 1;
+    const long long cw = 64ll * A.sm_spl;
+    const long long n_sc = ns > 0 ? (ns + cw - 1) / cw : 1;
     // items are decoded in 32 bits
     while (((total + sg - 1) / sg) * n_sc >= (1ll << 31)) sg *= 2;
     if (sg > total) sg = total > 0 ? total : 1;

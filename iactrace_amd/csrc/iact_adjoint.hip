@@ -348,7 +348,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       if (!SPLAT) {
         float gu_, gv_;
         const bool ok = plane_grid<KIND, false>(A.S, Kg, rf, gu_, gv_) && active;
-        const int pix = ok ? grid_pixel<KIND>(A.S, tab, gu_, gv_) : -1;
+        const int pix = grid_pixel<KIND>(A.S, tab, gu_, gv_, ok);
         gv = pix >= 0 ? G[pix] : 0.f;
       } else {
         float x, y;

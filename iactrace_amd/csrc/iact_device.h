@@ -458,8 +458,9 @@ struct DevSensor {
   float x0, y0, dx, dy, inv_dx, inv_dy, wf, hf, edge_width;
   // hex
   float size, inv_size, size_s3, size_15, inv_inradius, gcos, gsin, gox, goy, edge_thr;
-  float qmin_f, rmin_f, tq_f, tr_f;
+  float qmin_f, rmin_f, tq_f, tr_f;  // qmin_f = rmin_f = 0: axial is table-relative
   int q_min, r_min, tq, tr;
+  int edge;  // the sensor has an edge test (square edge_width > 0, hex edge_thr < 1)
   const int32_t* table;
   // splat
   float neg_half_inv_sigma2;
@@ -548,8 +549,37 @@ IACT_DEV int square_pixel_grid(const DevSensor& S, float xc, float yc) {
   const bool ok = (xf >= 0.f) & (xf < S.wf) & (yf >= 0.f) & (yf < S.hf);
   // row-major index in fp32: exact, the sensor having fewer than 2^24 pixels
   int pix = ok ? (int)fmaf(yf, S.wf, xf) : -1;
-  if (S.edge_width > 0.f) {
+  if (S.edge) {
     const float xr = xc - xf, yr = yc - yf;
+    const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
+    pix = dist < S.edge_width ? -1 : pix;
+  }
+  return pix;
+}
+
+// floor(x) converted to int32 in one instruction (v_cvt_flr_i32_f32; the
+// compiler forms it from floor + convert only when NaNs are ruled out for the
+// whole build).  Out-of-range values saturate and NaN converts to 0, so it
+// serves coordinates of rays that the caller has accepted or selects away.
+IACT_DEV int floor_i32(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// Square binning of the hit of a traced ray: the pixel of square_pixel_grid
+// for finite coordinates (a ray whose hit is not finite is rejected by the
+// trace itself), with floor and conversion in one instruction, the range
+// test as two unsigned compares and a 24-bit multiply-add for the index.
+// `valid` (the trace's own acceptance) joins the range test, so a rejected
+// ray costs no extra select.
+IACT_DEV int square_pixel_hit(const DevSensor& S, float xc, float yc, bool valid) {
+  const int ix = floor_i32(xc), iy = floor_i32(yc);
+  const bool ok =
+      valid & ((unsigned)ix < (unsigned)S.width) & ((unsigned)iy < (unsigned)S.height);
+  int pix = ok ? (int)(__umul24((unsigned)iy, (unsigned)S.width) + (unsigned)ix) : -1;
+  if (S.edge) {
+    const float xr = xc - floorf(xc), yr = yc - floorf(yc);
     const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
     pix = dist < S.edge_width ? -1 : pix;
   }
@@ -601,6 +631,34 @@ IACT_DEV int hex_pixel_grid(const DevSensor& S, const int32_t* table, float q, f
   const int pid = hex_lookup(S, table, qi, ri);
   const float hd = hex_norm_axial(q - qi, r - ri);
   return (pid >= 0 && !(hd > S.edge_thr)) ? pid : -1;
+}
+
+// float -> int32 with the hardware's saturation (v_cvt_i32_f32): out-of-range
+// values clamp to INT_MIN / INT_MAX and NaN converts to 0, where a C cast
+// would be undefined.
+IACT_DEV int trunc_i32(float x) {
+  int r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// Hard hex binning of a traced ray's hit (table-relative fractional axial
+// q, r), -1 when the trace rejected the ray (ok false): the pixel of
+// hex_pixel_grid, with the range test as two unsigned compares, the table
+// read for every lane (the index clamped into the table) and then selected,
+// and the edge test only for sensors with an edge.  Without one
+// (edge_thr = 1) the reference's test can only drop a point whose hex norm
+// rounds above 1, i.e. a point within rounding of a cell boundary.
+IACT_DEV int hex_pixel_hit(const DevSensor& S, const int32_t* table, float q, float r, bool ok) {
+  float qi, ri;
+  hex_round(q, r, qi, ri);
+  const int iq = trunc_i32(qi), ir = trunc_i32(ri);
+  const bool in = ((unsigned)iq < (unsigned)S.tq) & ((unsigned)ir < (unsigned)S.tr);
+  const unsigned last = __umul24((unsigned)S.tq, (unsigned)S.tr) - 1u;
+  const int pid = table[min(__umul24((unsigned)iq, (unsigned)S.tr) + (unsigned)ir, last)];
+  bool good = ok & in & (pid >= 0);
+  if (S.edge) good = good & !(hex_norm_axial(q - qi, r - ri) > S.edge_thr);
+  return good ? pid : -1;
 }
 
 IACT_DEV int hex_pixel(const DevSensor& S, const int32_t* table, float x, float y) {

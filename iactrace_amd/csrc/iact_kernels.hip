@@ -522,10 +522,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
           A.out[idx] = v;
           A.dbg_xy[2 * idx] = ok ? x : 1e10f;
           A.dbg_xy[2 * idx + 1] = ok ? y : 1e10f;
-          if (A.dbg_pix) A.dbg_pix[idx] = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
+          if (A.dbg_pix) A.dbg_pix[idx] = grid_pixel<KIND>(A.S, table, gu, gv, ok);
         }
       } else if (!SPLAT) {
-        const int pix = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
+        const int pix = grid_pixel<KIND>(A.S, table, gu, gv, ok);
         if (lds_image && acc == 1) {
           // per-lane run of equal pixels, flushed when the pixel changes
           if (pix != run_pix) {
@@ -634,16 +634,19 @@ __global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs
   }
 }
 
-// Image trace, source-major layout: lane = one source, a work item is a group
-// of `sample_group` consecutive samples (facet-major) traced against a chunk
-// of 64 sources; the sample's world-space record is wave-uniform (scalar
-// loads, prefetched one sample ahead).  Rays of a wave belong to different
-// sources and land on different pixels; with hard binning each lane sums its
-// consecutive rays that hit the same pixel (the samples of one facet put a
-// source's rays on one or two pixels) and adds the run to the LDS image when
-// the pixel changes; splat taps are added one by one.  The per-ray arithmetic
-// is trace_ray's, as in the sample-major kernel.
-template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST>
+// Image trace, source-major layout: lane = SPL sources (64 apart), a work
+// item is a group of `sample_group` consecutive samples (facet-major) traced
+// against a chunk of 64 SPL sources; the sample's world-space record is
+// wave-uniform (scalar loads, prefetched one sample ahead) and serves all
+// SPL sources of a lane, which halves the per-ray share of the record loads,
+// the loop control and the facet bookkeeping for SPL = 2.  Rays of a wave
+// belong to different sources and land on different pixels; with hard
+// binning each lane sums, per source, its consecutive rays that hit the same
+// pixel (the samples of one facet put a source's rays on one or two pixels)
+// and adds the run to the LDS image when the pixel changes; splat taps are
+// added one by one.  The per-ray arithmetic is trace_ray's, as in the
+// sample-major kernel.
+template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   extern __shared__ float smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
@@ -675,60 +678,88 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     // source chunk fastest: consecutive waves share the sample group
     const uint32_t g = fdiv((uint32_t)item, A.d_sc);
     const int c = (int)((uint32_t)item - g * A.n_sc);
-    const int s = c * 64 + lane;
     // lanes past the last source trace its rays with value 0 (their runs
     // are never added), which keeps every lane of the trace active
-    const int sl = s < A.ns ? s : A.ns - 1;
-    const double sx = A.sources[3 * sl], sy = A.sources[3 * sl + 1], sz = A.sources[3 * sl + 2];
-    const float value = s < A.ns ? A.values[sl] : 0.f;
+    double sx[SPL], sy[SPL], sz[SPL];
+    float value[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      const int s = (c * SPL + k) * 64 + lane;
+      const int sl = s < A.ns ? s : A.ns - 1;
+      sx[k] = A.sources[3 * sl];
+      sy[k] = A.sources[3 * sl + 1];
+      sz[k] = A.sources[3 * sl + 2];
+      value[k] = s < A.ns ? A.values[sl] : 0.f;
+    }
     // sample indices fit in 31 bits (checked on the host): scalar compares
     const int j0 = (int)g * A.sample_group;
     const int j1 = (int)min((long long)j0 + A.sample_group, total);
     const float4* recs = (const float4*)A.recs;
-    int run_pix = -1;
-    float run_v = 0.f;
-    // facet of the current samples, its obstruction candidates and the lane
-    // source's offset from it
+    int run_pix[SPL];
+    float run_v[SPL];
+    // facet of the current samples, its obstruction candidates (per chunk
+    // of 64 sources) and each lane source's offset from it
     int fend = -1;
     int fcur = 0;
-    FacetCull fc{0, 0u, 0};
-    f3 e = mk3(0.f, 0.f, 0.f);
+    FacetCull fc[SPL];
+    f3 e[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      run_pix[k] = -1;
+      run_v[k] = 0.f;
+      fc[k] = FacetCull{0, 0u, 0, 0};
+      e[k] = mk3(0.f, 0.f, 0.f);
+    }
     auto trace_one = [&](const RecRaw& rec, int j) {
       if (j >= fend) {
         fcur = (int)fdiv((uint32_t)j, A.d_m);
         fend = (fcur + 1) * A.M;
-        if (OBST) fc = facet_cull(A, fcur, c);
         const cptr<float> fp = as_const(A.positions) + 3 * fcur;  // scalar loads
         const f3 pos = mk3(fp[0], fp[1], fp[2]);
-        e = source_offset<SRC>(sx, sy, sz, pos);
-      }
-      Sample smp = unpack_record(rec);
-      fc.f = fcur;
-      set_facet(smp, fc);
-      float gu, gv, v, x, y;
-      const bool ok =
-          trace_ray<SRC, KIND, STAGES, AXIS, OBST>(A, smp, true, e, value, gu, gv, v, x, y);
-      if (!SPLAT) {
-        // run of equal pixels per lane (consecutive samples of a facet put a
-        // source's rays on the same one or two pixels), added when it ends;
-        // the pixel is found for every lane and then selected
-        const int pg = grid_pixel<KIND>(A.S, table, gu, gv);
-        const int pix = ok ? pg : -1;
-        if (pix != run_pix) {
-          if (run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
-          run_pix = pix;
-          run_v = 0.f;
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+          // (a chunk past the last source has value 0 everywhere: it reads
+          // the last chunk's row, which exists)
+          if (OBST) fc[k] = facet_cull(A, fcur, min(c * SPL + k, (A.ns - 1) >> 6));
+          e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
         }
-        run_v += v;  // a run of pix = -1 is never added
-      } else {
-        splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float w) {
-          const float vw = v * w;
-          if (pix >= 0 && vw != 0.f) add_px(pix, vw);
-        });
+      }
+      const Sample smp0 = unpack_record(rec);
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) {
+        Sample smp = smp0;
+        fc[k].f = fcur;
+        set_facet(smp, fc[k]);
+        float gu, gv, v, x, y, vf;
+        // a ray's value is linear in its source's: hard binning traces unit
+        // values and scales a run once, when it is added
+        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OBST>(
+            A, smp, true, e[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
+        if (!SPLAT) {
+          // run of equal pixels per lane and source (consecutive samples of
+          // a facet put a source's rays on the same one or two pixels), added
+          // when it ends
+          const int pix = grid_pixel<KIND>(A.S, table, gu, gv, ok);
+          if (pix != run_pix[k]) {
+            const float rv = value[k] * run_v[k];
+            if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
+            run_pix[k] = pix;
+            run_v[k] = 0.f;
+          }
+          // a run of pix = -1 is never added; without obstructions and
+          // stages v = vf * nrwh, summed in one fma
+          if (OBST || STAGES)
+            run_v[k] += v;
+          else
+            run_v[k] = fmaf(vf, smp.nrwh, run_v[k]);
+        } else {
+          splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float w) {
+            const float vw = v * w;
+            if (pix >= 0 && vw != 0.f) add_px(pix, vw);
+          });
+        }
       }
     };
-    // two record buffers used alternately (no register copies between
-    // iterations); each prefetches the record after the one being traced
     // the next record is loaded while one is traced (the record table has
     // padding at its end, so the prefetch needs no clamping)
     const float4* rp = recs + 4 * (size_t)j0;
@@ -739,7 +770,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       trace_one(cur, j);
       cur = nxt;
     }
-    if (!SPLAT && run_pix >= 0 && run_v != 0.f) add_px(run_pix, run_v);
+    if (!SPLAT) {
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) {
+        const float rv = value[k] * run_v[k];
+        if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
+      }
+    }
   }
   if (A.lds_image) {
     __syncthreads();
@@ -794,7 +831,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
       float gu, gv, v, x, y;
       const bool ok = trace_ray<SRC, KIND, true, false>(A, smp, active, e, val, gu, gv, v, x, y);
       if (!SPLAT) {
-        const int pix = ok ? grid_pixel<KIND>(A.S, table, gu, gv) : -1;
+        const int pix = grid_pixel<KIND>(A.S, table, gu, gv, ok);
         if (A.lds_image)
           wave_accumulate<false>(row, pix, v, A.max_rounds);
         else
@@ -868,13 +905,17 @@ static TraceKernel pick_trace_t(const TraceArgs& A) {
 
 // source-major kernels exist for primary-only scenes (the host selects that
 // layout only for those)
+template <int SRC, int KIND, int SPLAT, int SPL>
+static TraceKernel pick_sm_spl(const TraceArgs& A) {
+  if (has_obstructions(A.ob))
+    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL>
+                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL>;
+  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0, SPL>
+                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0, SPL>;
+}
 template <int SRC, int KIND, int SPLAT>
 static TraceKernel pick_sm_t(const TraceArgs& A) {
-  if (has_obstructions(A.ob))
-    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1>
-                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1>;
-  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0>
-                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0>;
+  return A.sm_spl == 2 ? pick_sm_spl<SRC, KIND, SPLAT, 2>(A) : pick_sm_spl<SRC, KIND, SPLAT, 1>(A);
 }
 
 template <int SRC>

@@ -98,7 +98,8 @@ struct TraceArgs {
   int layout;                    // 1 sample-major, 2 source-major
   const float* recs;             // (nf*M, IACT_REC_WORDS) world-space sample records
   int sample_group;              // samples per work item
-  uint32_t n_sc;                 // source chunks of 64
+  int sm_spl;                    // sources per lane (1 or 2)
+  uint32_t n_sc;                 // source chunks of 64 * sm_spl
   FastDiv d_sc;                  // divide by n_sc
   FastDiv d_m;                   // divide by M (sample -> facet)
   long long sm_items;            // sample groups * source chunks
@@ -111,9 +112,9 @@ struct TraceArgs {
 };
 
 // World-space sample record of the source-major trace (16 words, read with
-// broadcast vector loads): q = p - facet position, n, -1/weight, the
-// sensor-grid constants of p, p, and 3 words of padding -- the fields of the
-// Sample that make_sample builds which the image trace uses.
+// scalar loads): q = p - facet position, n, 1/(2 weight), the sensor-grid
+// constants of p, m = -2 n, p -- the fields of the Sample that make_sample
+// builds which the image trace uses.
 #define IACT_REC_WORDS 16
 
 // Position of a block in an XCD-major order.  Workgroups are dispatched to
@@ -219,8 +220,10 @@ IACT_DEV bool decode_item(const TraceArgs& A, long long item, int inst, int lane
 // One sample in world space, ready to be traced against many sources.
 struct Sample {
   f3 p, n;
+  f3 m;            // -2 n: the reflection's scaled cosine is one dot product
   f3 q;            // p - facet position, in fp32
   float nrw;       // -1 / weight (the reflected value is value * cos / w, cos = -d.n)
+  float nrwh;      // -nrw / 2 (value * (d.m) * nrwh is value * (d.n) * nrw exactly)
   PlaneK K;        // sensor-plane constants of the primary origin
   GridK G;         // the same in sensor-grid coordinates
   int f;           // facet (selects its obstruction candidate list)
@@ -270,8 +273,10 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_po
   Sample s;
   s.p = p;
   s.n = n;
+  s.m = -2.f * n;
   s.q = p - facet_pos;
   s.nrw = -frcp(w);
+  s.nrwh = -0.5f * s.nrw;
   s.K = plane_consts(S, p);
   s.G = grid_consts<KIND>(S, s.K);
   s.f = 0;
@@ -321,38 +326,40 @@ IACT_DEV void store_record(float* r, const Sample& s) {
   r[3] = s.n.x;
   r[4] = s.n.y;
   r[5] = s.n.z;
-  r[6] = s.nrw;
+  r[6] = s.nrwh;
   r[7] = s.G.kt;
   r[8] = s.G.ku;
   r[9] = s.G.kv;
-  r[10] = s.p.x;
-  r[11] = s.p.y;
-  r[12] = s.p.z;
-  r[13] = 0.f;
-  r[14] = 0.f;
-  r[15] = 0.f;
+  r[10] = s.m.x;
+  r[11] = s.m.y;
+  r[12] = s.m.z;
+  r[13] = s.p.x;
+  r[14] = s.p.y;
+  r[15] = s.p.z;
 }
 
-// Record read with scalar loads (the address is wave-uniform): 13 words into
-// SGPRs.  Vector loads of a uniform address would return the same 52 bytes
-// to each of the 64 lanes, and that data return, not the arithmetic, bounds
-// the trace; SGPR operands cost a move only where an instruction needs two.
+// Record read with scalar loads (the address is wave-uniform) into SGPRs;
+// words a trace does not use (p without obstructions) are not loaded.
+// Vector loads of a uniform address would return the same bytes to each of
+// the 64 lanes, and that data return, not the arithmetic, bounds the trace;
+// SGPR operands cost a move only where an instruction needs two.
 struct RecRaw {
-  float4 a, b, c;
-  float d;
+  float4 a, b, c, d;
 };
 IACT_DEV RecRaw fetch_record(const float4* r) {
   const cptr<float> q = as_const((const float*)r);
   return RecRaw{make_float4(q[0], q[1], q[2], q[3]), make_float4(q[4], q[5], q[6], q[7]),
-                make_float4(q[8], q[9], q[10], q[11]), q[12]};
+                make_float4(q[8], q[9], q[10], q[11]), make_float4(q[12], q[13], q[14], q[15])};
 }
 IACT_DEV Sample unpack_record(const RecRaw& q) {
   Sample s;
   s.q = mk3(q.a.x, q.a.y, q.a.z);
   s.n = mk3(q.a.w, q.b.x, q.b.y);
-  s.nrw = q.b.z;
+  s.nrwh = q.b.z;
+  s.nrw = -2.f * s.nrwh;
   s.G = GridK{q.b.w, q.c.x, q.c.y};
-  s.p = mk3(q.c.z, q.c.w, q.d);
+  s.m = mk3(q.c.z, q.c.w, q.d.x);
+  s.p = mk3(q.d.y, q.d.z, q.d.w);
   s.K = PlaneK{0.f, 0.f, 0.f};  // world hit coordinates are not traced from records
   s.f = 0;
   s.cand0 = 0u;
@@ -511,12 +518,19 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 // sensor frame.
 template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float value,
-                        float& gu, float& gv, float& v, float& x, float& y) {
+                        float& gu, float& gv, float& v, float& x, float& y,
+                        float* vfac = nullptr) {
   float inv;
   const f3 d = primary_direction<SRC>(s, e, inv);
-  const float c = dot(d, s.n);
-  f3 r = fma3(-2.f * c, s.n, d);
-  v = value * c * s.nrw;
+  // c2 = d.m = -2 (d.n) and value * c2 * nrwh = value * (d.n) * nrw, both
+  // exactly (scalings by powers of two).  Traces with obstructions form c2
+  // from n, leaving m unread: their scalar registers are the scarcer resource.
+  const float c2 = OBST == 0 ? dot(d, s.m) : -2.f * dot(d, s.n);
+  f3 r = fma3(c2, s.n, d);
+  // vfac: the value before the weight, v = vfac * nrwh without obstructions
+  // or further stages
+  if (vfac) *vfac = value * c2;
+  v = value * c2 * s.nrwh;
   v *= primary_shadow<OBST>(A, s, d, active);
   PlaneK K = s.K;
   GridK G = s.G;
@@ -532,11 +546,12 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
   return ok && active;
 }
 
-// Hard pixel of a hit in sensor-grid coordinates; -1 when dropped.
+// Hard pixel of a traced ray's hit in sensor-grid coordinates; -1 when
+// dropped or when the trace rejected the ray (ok false).
 template <int KIND>
-IACT_DEV int grid_pixel(const DevSensor& S, const int32_t* table, float gu, float gv) {
-  if (KIND == IACT_SENSOR_SQUARE) return square_pixel_grid(S, gu, gv);
-  return hex_pixel_grid(S, table, gu, gv);
+IACT_DEV int grid_pixel(const DevSensor& S, const int32_t* table, float gu, float gv, bool ok) {
+  if (KIND == IACT_SENSOR_SQUARE) return square_pixel_hit(S, gu, gv, ok);
+  return hex_pixel_hit(S, table, gu, gv, ok);
 }
 
 // Hard pixel of a hit (x, y) in the sensor frame.
