@@ -2,6 +2,7 @@
 // and response-matrix trace.  Host launchers at the bottom are called by the
 // C ABI in iact_abi.hip.
 #include "iact_kernels.h"
+#include <type_traits>
 
 namespace iact {
 
@@ -238,8 +239,10 @@ IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
 }
 
 // Can a ray of the bundle reach obstruction j (mask numbering: cylinders,
-// boxes, spheres, oriented boxes, triangles)?  Long cylinders are tested as
-// a chain of bounding spheres of pieces at most 1 m long (at most 64).
+// boxes, spheres, oriented boxes, triangles)?  Cylinders are tested as a
+// chain of bounding spheres of pieces as long as the radius (at most 64
+// pieces): a piece's sphere is then 1.12 radii across where a 1 m piece of a
+// 0.1 m mast would be 5.1, so the masks keep far fewer facets per mast.
 IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
                                   float tn) {
   const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
@@ -248,7 +251,9 @@ IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, floa
     const float* q = ob.cylinders + 8 * j;
     const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
     const float h = q[6], r = q[7];
-    const int n = (int)fminf(fmaxf(ceilf(h), 1.f), 64.f);
+    // (NaN or zero radii give one piece)
+    const float per = h / fmaxf(r, 1e-6f);
+    const int n = per > 1.f ? (int)fminf(ceilf(per), 64.f) : 1;
     const float hp = h / (float)n;
     const float R = sqrtf(0.25f * hp * hp + r * r);
     for (int i = 0; i < n; ++i)
@@ -645,8 +650,12 @@ __global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs
 // pixel (the samples of one facet put a source's rays on one or two pixels)
 // and adds the run to the LDS image when the pixel changes; splat taps are
 // added one by one.  The per-ray arithmetic is trace_ray's, as in the
-// sample-major kernel.
-template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL>
+// sample-major kernel.  In obstructed scenes (OBST) the work splits by the
+// facet runs of the items: PART 1 traces the runs without an obstruction
+// candidate for any of the lane's chunks (most of them when culling), PART 2
+// those with one, with shadow tests; the host launches both over the same
+// items, so each kernel's registers are allocated for its own loop.
+template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL, int PART>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   extern __shared__ float smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
@@ -697,43 +706,29 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const float4* recs = (const float4*)A.recs;
     int run_pix[SPL];
     float run_v[SPL];
-    // facet of the current samples, its obstruction candidates (per chunk
-    // of 64 sources) and each lane source's offset from it
-    int fend = -1;
-    int fcur = 0;
+    // the current facet's obstruction candidates (per chunk of 64 sources)
+    // and each lane source's offset from it
     FacetCull fc[SPL];
     f3 e[SPL];
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
       run_pix[k] = -1;
       run_v[k] = 0.f;
-      fc[k] = FacetCull{0, 0u, 0, 0};
-      e[k] = mk3(0.f, 0.f, 0.f);
     }
-    auto trace_one = [&](const RecRaw& rec, int j) {
-      if (j >= fend) {
-        fcur = (int)fdiv((uint32_t)j, A.d_m);
-        fend = (fcur + 1) * A.M;
-        const cptr<float> fp = as_const(A.positions) + 3 * fcur;  // scalar loads
-        const f3 pos = mk3(fp[0], fp[1], fp[2]);
-#pragma unroll
-        for (int k = 0; k < SPL; ++k) {
-          // (a chunk past the last source has value 0 everywhere: it reads
-          // the last chunk's row, which exists)
-          if (OBST) fc[k] = facet_cull(A, fcur, min(c * SPL + k, (A.ns - 1) >> 6));
-          e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
-        }
-      }
+    // one sample's rays, with shadow tests (SH) or without; the shadow
+    // kernel forms the reflection from n (its records' m words stay unread:
+    // its scalar registers are the scarcer resource)
+    auto trace_one = [&](auto sh, const RecRaw& rec) {
+      constexpr int OB = decltype(sh)::value ? OBST : 0;
       const Sample smp0 = unpack_record(rec);
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
         Sample smp = smp0;
-        fc[k].f = fcur;
         set_facet(smp, fc[k]);
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
-        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OBST>(
+        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0>(
             A, smp, true, e[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
         if (!SPLAT) {
           // run of equal pixels per lane and source (consecutive samples of
@@ -746,9 +741,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
             run_pix[k] = pix;
             run_v[k] = 0.f;
           }
-          // a run of pix = -1 is never added; without obstructions and
+          // a run of pix = -1 is never added; without shadow tests and
           // stages v = vf * nrwh, summed in one fma
-          if (OBST || STAGES)
+          if (OB || STAGES)
             run_v[k] += v;
           else
             run_v[k] = fmaf(vf, smp.nrwh, run_v[k]);
@@ -760,15 +755,46 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         }
       }
     };
-    // the next record is loaded while one is traced (the record table has
-    // padding at its end, so the prefetch needs no clamping)
-    const float4* rp = recs + 4 * (size_t)j0;
-    RecRaw cur = fetch_record(rp);
-    for (int j = j0; j < j1; ++j) {
-      rp += 4;
-      const RecRaw nxt = fetch_record(rp);
-      trace_one(cur, j);
-      cur = nxt;
+    for (int j = j0; j < j1;) {
+      // the run of the group's samples on one facet
+      const int f = (int)fdiv((uint32_t)j, A.d_m);
+      const int jend = min((f + 1) * A.M, j1);
+      int any = 0;
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) {
+        // (a chunk past the last source has value 0 everywhere: it reads
+        // the last chunk's row, which exists)
+        fc[k] = OBST ? facet_cull(A, f, min(c * SPL + k, (A.ns - 1) >> 6)) : FacetCull{f, 0u, 0, 0};
+        any |= fc[k].any;
+      }
+      const bool shadow = OBST && any;
+      if (OBST && (PART == 1 ? shadow : !shadow)) {  // the other kernel's run
+        j = jend;
+        continue;
+      }
+      const cptr<float> fp = as_const(A.positions) + 3 * f;  // scalar loads
+      const f3 pos = mk3(fp[0], fp[1], fp[2]);
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+      // the next record is loaded while one is traced (the record table has
+      // padding at its end, so the prefetch needs no clamping)
+      const float4* rp = recs + 4 * (size_t)j;
+      RecRaw cur = fetch_record(rp);
+      if (OBST && PART == 2) {
+        for (; j < jend; ++j) {
+          rp += 4;
+          const RecRaw nxt = fetch_record(rp);
+          trace_one(std::true_type{}, cur);
+          cur = nxt;
+        }
+      } else {
+        for (; j < jend; ++j) {
+          rp += 4;
+          const RecRaw nxt = fetch_record(rp);
+          trace_one(std::false_type{}, cur);
+          cur = nxt;
+        }
+      }
     }
     if (!SPLAT) {
 #pragma unroll
@@ -904,18 +930,25 @@ static TraceKernel pick_trace_t(const TraceArgs& A) {
 }
 
 // source-major kernels exist for primary-only scenes (the host selects that
-// layout only for those)
+// layout only for those); obstructed scenes take two kernels, for the runs
+// with obstruction candidates (TRACE_SOURCE_MAJOR) and those without
+// (TRACE_SOURCE_MAJOR_FREE)
 template <int SRC, int KIND, int SPLAT, int SPL>
-static TraceKernel pick_sm_spl(const TraceArgs& A) {
-  if (has_obstructions(A.ob))
-    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL>
-                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL>;
-  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0, SPL>
-                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0, SPL>;
+static TraceKernel pick_sm_spl(const TraceArgs& A, int which) {
+  if (has_obstructions(A.ob)) {
+    if (which == TRACE_SOURCE_MAJOR_FREE)
+      return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL, 1>
+                      : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL, 1>;
+    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL, 2>
+                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL, 2>;
+  }
+  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0, SPL, 0>
+                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0, SPL, 0>;
 }
 template <int SRC, int KIND, int SPLAT>
-static TraceKernel pick_sm_t(const TraceArgs& A) {
-  return A.sm_spl == 2 ? pick_sm_spl<SRC, KIND, SPLAT, 2>(A) : pick_sm_spl<SRC, KIND, SPLAT, 1>(A);
+static TraceKernel pick_sm_t(const TraceArgs& A, int which) {
+  return A.sm_spl == 2 ? pick_sm_spl<SRC, KIND, SPLAT, 2>(A, which)
+                       : pick_sm_spl<SRC, KIND, SPLAT, 1>(A, which);
 }
 
 template <int SRC>
@@ -932,9 +965,11 @@ static TraceKernel pick_src(const TraceArgs& A, int which) {
       return sq ? pick_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_DEBUG>(A)
                 : pick_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_DEBUG>(A);
     case TRACE_SOURCE_MAJOR:
-      if (sq) return sp ? pick_sm_t<SRC, IACT_SENSOR_SQUARE, 1>(A)
-                        : pick_sm_t<SRC, IACT_SENSOR_SQUARE, 0>(A);
-      return sp ? pick_sm_t<SRC, IACT_SENSOR_HEX, 1>(A) : pick_sm_t<SRC, IACT_SENSOR_HEX, 0>(A);
+    case TRACE_SOURCE_MAJOR_FREE:
+      if (sq) return sp ? pick_sm_t<SRC, IACT_SENSOR_SQUARE, 1>(A, which)
+                        : pick_sm_t<SRC, IACT_SENSOR_SQUARE, 0>(A, which);
+      return sp ? pick_sm_t<SRC, IACT_SENSOR_HEX, 1>(A, which)
+                : pick_sm_t<SRC, IACT_SENSOR_HEX, 0>(A, which);
     default:
       if (sq) return sp ? pick_trace_t<SRC, IACT_SENSOR_SQUARE, 1, IACT_MODE_IMAGE>(A)
                         : pick_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_IMAGE>(A);
@@ -967,6 +1002,13 @@ hipError_t launch_trace(const TraceArgs& A, const LaunchShape& L, int source_typ
 
 hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_type,
                            hipStream_t st) {
+  // obstructed scenes: the runs without candidates first (none when every
+  // mask bit is set), then those with
+  if (has_obstructions(A.ob) && !A.cull_all) {
+    const hipError_t e =
+        launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR_FREE), A, L, st);
+    if (e != hipSuccess) return e;
+  }
   return launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR), A, L, st);
 }
 

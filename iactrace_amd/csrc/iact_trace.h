@@ -516,16 +516,18 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 // inactive; v is 0 for inactive lanes.  (gu, gv) is the hit in sensor-grid
 // coordinates (what the binning reads); XY also gives the hit (x, y) in the
 // sensor frame.
-template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false>
+template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false,
+          bool RM = (OBST == 0)>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float value,
                         float& gu, float& gv, float& v, float& x, float& y,
                         float* vfac = nullptr) {
   float inv;
   const f3 d = primary_direction<SRC>(s, e, inv);
   // c2 = d.m = -2 (d.n) and value * c2 * nrwh = value * (d.n) * nrw, both
-  // exactly (scalings by powers of two).  Traces with obstructions form c2
-  // from n, leaving m unread: their scalar registers are the scarcer resource.
-  const float c2 = OBST == 0 ? dot(d, s.m) : -2.f * dot(d, s.n);
+  // exactly (scalings by powers of two).  RM false forms c2 from n, leaving
+  // m unread (traces with obstructions: their scalar registers are the
+  // scarcer resource).
+  const float c2 = RM ? dot(d, s.m) : -2.f * dot(d, s.n);
   f3 r = fma3(c2, s.n, d);
   // vfac: the value before the weight, v = vfac * nrwh without obstructions
   // or further stages
