@@ -271,21 +271,25 @@ IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
   const f3 ax = mk3(c[3], c[4], c[5]);
   const float h = c[6], r = c[7];
   const f3 oc = o - p1;
+  const float inf = __builtin_inff();
+  // Early out.  With w = d x ax, (oc.w)^2 / |w|^2 is the squared distance of
+  // the ray's line from the axis; when it exceeds r^2 by 2e-4 r^2 plus
+  // 1e-5 |oc|^2 (a bound on the rounding of this estimate and of the
+  // discriminant below, which has the same sign as r^2 minus that distance)
+  // no side or cap hit is possible.  Lines parallel to the axis (w = 0) and
+  // NaN geometry never take it.  The per-lane result is the miss for such a
+  // lane whatever its wave, and the rest of the test is skipped when it
+  // holds for the whole wave.
+  const f3 w = cross(d, ax);
+  const float ow = dot(oc, w);
+  const bool far = ow * ow > dot(w, w) * fmaf(1e-5f, dot(oc, oc), 1.0002f * r * r);
+  if (__all(far)) return inf;
   const float oca = dot(oc, ax), rda = dot(d, ax);
   const f3 ocp = oc - oca * ax, rdp = d - rda * ax;
   const float a = dot(rdp, rdp);
   const float b = 2.f * dot(ocp, rdp);
   const float cc = dot(ocp, ocp) - r * r;
   const float disc = b * b - 4.f * a * cc;
-  const float inf = __builtin_inff();
-  // Early out.  -disc / (4a) is the squared distance of the ray's line from
-  // the axis minus r^2; when it exceeds ~1e-4 r (the line passes more than
-  // ~5e-5 m outside the surface) plus a bound on the rounding of disc, no
-  // side or cap hit is possible.  The per-lane result is the miss for such a
-  // lane whatever its wave, and the rest of the test is skipped when it
-  // holds for the whole wave.
-  const bool far = -disc > fmaf(4.f * a, 1e-4f * r, 1e-6f * fmaf(4.f * a, fabsf(cc), b * b));
-  if (__all(far)) return inf;
   const float sq = fsqrt(fmaxf(disc, 0.f));
   const float ia = frcp(2.f * a + eps);
   float t1 = (-b - sq) * ia, t2 = (-b + sq) * ia;
@@ -405,6 +409,15 @@ IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool activ
 // once, before the per-ray set-up that the compiler hoists out of the tests,
 // and without a scalar-memory wait (which would also wait for the caller's
 // prefetches).
+// A copy of v that the compiler cannot trace back to v: arithmetic on it
+// stays in the branch that makes the copy, where it would otherwise be
+// hoisted out of the candidate loop and run for every ray with candidates,
+// whichever obstruction kinds they are.
+IACT_DEV f3 pinned(f3 v) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
+  return v;
+}
+
 IACT_DEV float shadow_factor_culled(const iact_obstructions& ob, cptr<uint32_t> mask, int words,
                                     uint32_t first, int any, f3 o, f3 d, bool active) {
   const float big = 1e10f;
@@ -422,13 +435,13 @@ IACT_DEV float shadow_factor_culled(const iact_obstructions& ob, cptr<uint32_t> 
       if (j < c0)
         t = hit_cylinder(o, d, as_const(ob.cylinders) + 8 * j);
       else if (j < c1)
-        t = hit_box(o, d, as_const(ob.boxes) + 6 * (j - c0));
+        t = hit_box(o, pinned(d), as_const(ob.boxes) + 6 * (j - c0));
       else if (j < c2)
-        t = hit_sphere(o, d, as_const(ob.spheres) + 4 * (j - c1));
+        t = hit_sphere(o, pinned(d), as_const(ob.spheres) + 4 * (j - c1));
       else if (j < c3)
-        t = hit_oriented_box(o, d, as_const(ob.oriented_boxes) + 15 * (j - c2));
+        t = hit_oriented_box(o, pinned(d), as_const(ob.oriented_boxes) + 15 * (j - c2));
       else
-        t = hit_triangle(o, d, as_const(ob.triangles) + 9 * (j - c3));
+        t = hit_triangle(o, pinned(d), as_const(ob.triangles) + 9 * (j - c3));
       blocked |= t < big;
     }
   }
