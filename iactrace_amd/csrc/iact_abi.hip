@@ -200,6 +200,18 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
     S.ary = (float)ary;
     S.ar0 = (float)(-(arx * ox + ary * oy) - (double)s.r_min);
   }
+  // bounds for plane_test_redundant (grid units): the smallest stretch of an
+  // in-plane direction by the grid map, and the grid's extent
+  if (s.kind == IACT_SENSOR_SQUARE) {
+    S.gscale = fminf(1.f, S.v_scale);
+    S.gext = (float)s.width + (float)s.height + 4.f;
+  } else {
+    const double a = S.aqx, b = S.aqy, c = S.arx, d = S.ary;
+    const double F = a * a + b * b + c * c + d * d, D = fabs(a * d - b * c);
+    const double smin2 = 0.5 * (F - sqrt(fmax(F * F - 4.0 * D * D, 0.0)));
+    S.gscale = (float)(0.999 * sqrt(fmax(smin2, 0.0)));
+    S.gext = (float)s.table_q + (float)s.table_r + 4.f;
+  }
   S.edge_thr = s.edge_threshold;
   S.q_min = s.q_min;
   S.r_min = s.r_min;

@@ -347,7 +347,16 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       float gv, gx = 0.f, gy = 0.f;
       if (!SPLAT) {
         float gu_, gv_;
-        const bool ok = plane_grid<KIND, false>(A.S, Kg, rf, gu_, gv_) && active;
+        bool ok;
+        if (SRC == IACT_SOURCE_POINT && (!STAGES || A.n_stages == 0)) {
+          // the hit as trace_ray forms it (the reflection of u = q + e, not
+          // normalised), so that the pixel is the forward's
+          const f3 u = smp.q + e;
+          const f3 ru = fma3(-2.f * dot(u, smp.n), smp.n, u);
+          ok = plane_grid<KIND, false>(A.S, Kg, ru, gu_, gv_, inv) && active;
+        } else {
+          ok = plane_grid<KIND, false>(A.S, Kg, rf, gu_, gv_) && active;
+        }
         const int pix = grid_pixel<KIND>(A.S, tab, gu_, gv_, ok);
         gv = pix >= 0 ? G[pix] : 0.f;
       } else {

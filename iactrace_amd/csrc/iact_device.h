@@ -474,6 +474,10 @@ struct DevSensor {
   float qmin_f, rmin_f, tq_f, tr_f;  // qmin_f = rmin_f = 0: axial is table-relative
   int q_min, r_min, tq, tr;
   int edge;  // the sensor has an edge test (square edge_width > 0, hex edge_thr < 1)
+  // A ray from origin constants G reaching the plane at t = G.kt / r_n lands
+  // at least |G.kt| gscale |r_inplane| / |r_n| (grid units) from (G.ku, G.kv);
+  // any hit on the grid's table lies within |ku| + |kv| + gext of it.
+  float gscale, gext;
   const int32_t* table;
   // splat
   float neg_half_inv_sigma2;
@@ -495,7 +499,10 @@ IACT_DEV PlaneK plane_consts(const DevSensor& S, f3 o) {
 // Returns false for rays parallel to the plane or hitting it at t <= 0, the
 // reference's 1e10 sentinel cases.
 template <bool AXIS>
-IACT_DEV bool plane_xy(const DevSensor& S, const PlaneK& K, f3 r, float& x, float& y) {
+// rs: 1/|r| when r is not a unit vector (the hit depends only on r's
+// direction; the parallel-ray test is on the unit vector's normal component).
+IACT_DEV bool plane_xy(const DevSensor& S, const PlaneK& K, f3 r, float& x, float& y,
+                       float rs = 1.f) {
   float nd, ru, rv;
   if (AXIS) {
     nd = r.z;
@@ -509,7 +516,7 @@ IACT_DEV bool plane_xy(const DevSensor& S, const PlaneK& K, f3 r, float& x, floa
   const float t = K.kt * frcp(nd);
   x = fmaf(t, ru, K.kx);
   y = fmaf(t, rv, K.ky);
-  return !(fabsf(nd) < 1e-10f) && t > 0.f;
+  return !(fabsf(nd * rs) < 1e-10f) && t > 0.f;
 }
 
 // Sensor-grid constants of an origin.  The plane hit (x, y) = (kx + t ru,
@@ -530,8 +537,11 @@ IACT_DEV GridK grid_consts(const DevSensor& S, const PlaneK& K) {
 }
 
 // plane_xy in grid coordinates (gu, gv); the same miss conditions.
-template <int KIND, bool AXIS>
-IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, float& gv) {
+// PAR false leaves out the parallel-ray test, for origins where a ray that
+// fails it lands off the grid anyway (plane_test_redundant).
+template <int KIND, bool AXIS, bool PAR = true>
+IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, float& gv,
+                         float rs = 1.f) {
   float nd, ru, rv;
   if (AXIS) {
     nd = r.z;
@@ -550,7 +560,15 @@ IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, fl
     gu = fmaf(t, fmaf(ru, S.aqx, rv * S.aqy), G.ku);
     gv = fmaf(t, fmaf(ru, S.arx, rv * S.ary), G.kv);
   }
-  return !(fabsf(nd) < 1e-10f) && t > 0.f;
+  return (!PAR || !(fabsf(nd * rs) < 1e-10f)) && t > 0.f;
+}
+
+// True when the parallel-ray test (|r_n| < 1e-10 for a unit r) cannot change
+// a binning decision for rays from origin constants G: such a ray lands at
+// least 0.99e10 |G.kt| gscale from (ku, kv), and 1e9 |G.kt| gscale already
+// exceeds the farthest grid position; NaN constants keep the test.
+IACT_DEV bool plane_test_redundant(const DevSensor& S, const GridK& G) {
+  return fabsf(G.kt) * S.gscale * 1e9f > fabsf(G.ku) + fabsf(G.kv) + S.gext;
 }
 
 // Square binning (square.py:66-91) of pixel coordinates (xc, yc): floor,

@@ -590,8 +590,12 @@ IACT_DEV uint32_t spread4(uint32_t v) {  // 4 bits -> even positions
   return v;
 }
 
+// World-space records of every sample, per facet sorted by a 16 x 16 Morton
+// cell of the aperture position; *flag (zeroed by the launcher) becomes
+// non-zero when some sample needs the parallel-ray test.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs) {
+__global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs,
+                                                        uint32_t* flag) {
   __shared__ uint32_t cnt[256];
   __shared__ float ext[256];
   const int t = threadIdx.x;
@@ -632,8 +636,10 @@ __global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs
       const uint32_t slot = atomicAdd(cnt + key(m), 1u);
       f3 pw, nw;
       world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], g0 + m, pw, nw);
-      store_record(recs + IACT_REC_WORDS * (g0 + slot),
-                   make_sample<KIND>(A.S, pw, nw, A.weights[g0 + m], pos));
+      const Sample smp = make_sample<KIND>(A.S, pw, nw, A.weights[g0 + m], pos);
+      store_record(recs + IACT_REC_WORDS * (g0 + slot), smp);
+      // a sample whose rays need the parallel-ray test marks the table
+      if (!plane_test_redundant(A.S, smp.G)) atomicOr(flag, 1u);
     }
     __syncthreads();
   }
@@ -682,6 +688,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       unsafeAtomicAdd(A.out + pix, v);
   };
   const long long total = (long long)A.nf * A.M;
+  // no sample needs the parallel-ray test (the flag after the records)
+  const bool par_free = as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u;
   for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
        item += (long long)gridDim.x * wpb) {
     // source chunk fastest: consecutive waves share the sample group
@@ -718,8 +726,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     // one sample's rays, with shadow tests (SH) or without; the shadow
     // kernel forms the reflection from n (its records' m words stay unread:
     // its scalar registers are the scarcer resource)
-    auto trace_one = [&](auto sh, const RecRaw& rec) {
+    auto trace_one = [&](auto sh, auto par, const RecRaw& rec) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
+      constexpr bool PAR = decltype(par)::value;
       const Sample smp0 = unpack_record(rec);
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
@@ -728,7 +737,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
-        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0>(
+        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0, PAR>(
             A, smp, true, e[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
         if (!SPLAT) {
           // run of equal pixels per lane and source (consecutive samples of
@@ -780,20 +789,23 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       // padding at its end, so the prefetch needs no clamping)
       const float4* rp = recs + 4 * (size_t)j;
       RecRaw cur = fetch_record(rp);
+      auto run = [&](auto sh, auto par) {
+        for (; j < jend; ++j) {
+          rp += 4;
+          const RecRaw nxt = fetch_record(rp);
+          trace_one(sh, par, cur);
+          cur = nxt;
+        }
+      };
       if (OBST && PART == 2) {
-        for (; j < jend; ++j) {
-          rp += 4;
-          const RecRaw nxt = fetch_record(rp);
-          trace_one(std::true_type{}, cur);
-          cur = nxt;
-        }
+        // (one loop: the shadow tests dominate, and a second copy would cost
+        // this kernel scalar registers)
+        run(std::true_type{}, std::true_type{});
       } else {
-        for (; j < jend; ++j) {
-          rp += 4;
-          const RecRaw nxt = fetch_record(rp);
-          trace_one(std::false_type{}, cur);
-          cur = nxt;
-        }
+        if (par_free)
+          run(std::false_type{}, std::false_type{});
+        else
+          run(std::false_type{}, std::true_type{});
       }
     }
     if (!SPLAT) {
@@ -1037,12 +1049,15 @@ hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_s
 
 hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st) {
   const long long total = (long long)A.nf * A.M;
+  // the flag word is the first word of the padding after the records
+  float* flag = recs + IACT_REC_WORDS * total;
+  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st, flag, 1ll, 0.f);
   if (A.S.kind == IACT_SENSOR_SQUARE)
     hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_SQUARE>, dim3(grid_for(A.nf, 1, 16384)),
-                       dim3(256), 0, st, A, recs);
+                       dim3(256), 0, st, A, recs, (uint32_t*)flag);
   else
     hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_HEX>, dim3(grid_for(A.nf, 1, 16384)),
-                       dim3(256), 0, st, A, recs);
+                       dim3(256), 0, st, A, recs, (uint32_t*)flag);
   return hipGetLastError();
 }
 

@@ -511,24 +511,44 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
+// PAR false (image traces of samples whose plane_test_redundant holds) leaves
+// out the parallel-ray test; ok may then hold for a ray that lands far off
+// the grid, which binning drops.
 // e is the facet's source offset (source_offset).  Returns false when the
 // ray misses the plane (the reference's 1e10 sentinel) or the lane is
 // inactive; v is 0 for inactive lanes.  (gu, gv) is the hit in sensor-grid
 // coordinates (what the binning reads); XY also gives the hit (x, y) in the
 // sensor frame.
 template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false,
-          bool RM = (OBST == 0)>
+          bool RM = (OBST == 0), bool PAR = true>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float value,
                         float& gu, float& gv, float& v, float& x, float& y,
                         float* vfac = nullptr) {
-  float inv;
-  const f3 d = primary_direction<SRC>(s, e, inv);
-  // c2 = d.m = -2 (d.n) and value * c2 * nrwh = value * (d.n) * nrw, both
-  // exactly (scalings by powers of two).  RM false forms c2 from n, leaving
-  // m unread (traces with obstructions: their scalar registers are the
-  // scarcer resource).
-  const float c2 = RM ? dot(d, s.m) : -2.f * dot(d, s.n);
-  f3 r = fma3(c2, s.n, d);
+  // With m = -2 n, d.m = -2 (d.n) and value * (d.m) * nrwh = value * (d.n) *
+  // nrw, both exactly (scalings by powers of two).  RM false forms d.m from
+  // n, leaving m unread (traces with obstructions: their scalar registers are
+  // the scarcer resource).
+  float inv, c2, rs = 1.f;
+  f3 d, r;
+  // (every kernel takes this path when the scene has no further stage, so
+  // that all of them bin a ray alike)
+  if (SRC == IACT_SOURCE_POINT && (!STAGES || A.n_stages == 0)) {
+    // The reflection of u = q + e, r = u + (u.m) n, is |u| times the unit
+    // reflection, and the sensor hit depends only on its direction: u is not
+    // normalised, 1/|u| enters the cosine and the parallel-ray test only
+    // (the unit direction d is formed for the shadow test alone).
+    const f3 u = s.q + e;
+    inv = frsq(dot(u, u));
+    const float c2u = RM ? dot(u, s.m) : -2.f * dot(u, s.n);
+    r = fma3(c2u, s.n, u);
+    c2 = c2u * inv;
+    rs = inv;
+    d = inv * u;
+  } else {
+    d = primary_direction<SRC>(s, e, inv);
+    c2 = RM ? dot(d, s.m) : -2.f * dot(d, s.n);
+    r = fma3(c2, s.n, d);
+  }
   // vfac: the value before the weight, v = vfac * nrwh without obstructions
   // or further stages
   if (vfac) *vfac = value * c2;
@@ -542,8 +562,8 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
     K = plane_consts(A.S, o);
     G = grid_consts<KIND>(A.S, K);
   }
-  const bool ok = plane_grid<KIND, AXIS>(A.S, G, r, gu, gv);
-  if (XY) plane_xy<AXIS>(A.S, K, r, x, y);
+  const bool ok = plane_grid<KIND, AXIS, PAR || STAGES>(A.S, G, r, gu, gv, rs);
+  if (XY) plane_xy<AXIS>(A.S, K, r, x, y, rs);
   if (!active) v = 0.f;
   return ok && active;
 }

@@ -105,6 +105,24 @@ def test_layouts_c3_hex_obstructions(layout):
                        launch=ia.LaunchConfig(layout=layout, sample_group=5))
 
 
+@pytest.mark.parametrize("layout", [1, 2])
+def test_facet_in_the_sensor_plane(layout):
+    """A flat, untilted facet lying in the sensor plane (its samples are at
+    distance 0 from it): the source-major trace then keeps the parallel-ray
+    test for every sample, which it otherwise leaves to the binning; the
+    image still matches the oracle."""
+    from iactrace_amd.configs import AsphericSurface, Mirror, PolygonAperture, hex_facet_vertices
+    wl = configs.workload("C2", n_sources=80, n_samples=16)
+    mirrors = configs.davies_cotton_dish(n_facets=19) + [
+        Mirror(np.array([5.0, 0.0, 15.0]), np.zeros(3), AsphericSurface(0.0, 0.0, []),
+               PolygonAperture(hex_facet_vertices(0.6)))]
+    w2 = configs.Workload("plane", mirrors, [], wl.sensor, 16, wl.sources, wl.values, "point")
+    tel = w2.telescope(DEV, key=3)
+    img, _ = _trace_and_compare(tel, w2.sources, w2.values, "point", f"in-plane facet {layout}",
+                                launch=ia.LaunchConfig(layout=layout))
+    assert img.sum() > 0
+
+
 @pytest.mark.parametrize("kind", ["square", "hex"])
 def test_layouts_agree_on_splat_and_parallel(kind):
     """The two layouts trace identical rays; only the order of the float
