@@ -604,12 +604,14 @@ IACT_DEV int floor_i32(float x) {
 // test as two unsigned compares and a 24-bit multiply-add for the index.
 // `valid` (the trace's own acceptance) joins the range test, so a rejected
 // ray costs no extra select.
+// EDGE false: the caller knows that the sensor has no edge test.
+template <bool EDGE = true>
 IACT_DEV int square_pixel_hit(const DevSensor& S, float xc, float yc, bool valid) {
   const int ix = floor_i32(xc), iy = floor_i32(yc);
   const bool ok =
       valid & ((unsigned)ix < (unsigned)S.width) & ((unsigned)iy < (unsigned)S.height);
   int pix = ok ? (int)(__umul24((unsigned)iy, (unsigned)S.width) + (unsigned)ix) : -1;
-  if (S.edge) {
+  if (EDGE && S.edge) {
     const float xr = xc - floorf(xc), yr = yc - floorf(yc);
     const float dist = fminf(fminf(xr, 1.f - xr) * S.dx, fminf(yr, 1.f - yr) * S.dy);
     pix = dist < S.edge_width ? -1 : pix;
@@ -680,6 +682,7 @@ IACT_DEV int trunc_i32(float x) {
 // and the edge test only for sensors with an edge.  Without one
 // (edge_thr = 1) the reference's test can only drop a point whose hex norm
 // rounds above 1, i.e. a point within rounding of a cell boundary.
+template <bool EDGE = true>
 IACT_DEV int hex_pixel_hit(const DevSensor& S, const int32_t* table, float q, float r, bool ok) {
   float qi, ri;
   hex_round(q, r, qi, ri);
@@ -688,7 +691,7 @@ IACT_DEV int hex_pixel_hit(const DevSensor& S, const int32_t* table, float q, fl
   const unsigned last = __umul24((unsigned)S.tq, (unsigned)S.tr) - 1u;
   const int pid = table[min(__umul24((unsigned)iq, (unsigned)S.tr) + (unsigned)ir, last)];
   bool good = ok & in & (pid >= 0);
-  if (S.edge) good = good & !(hex_norm_axial(q - qi, r - ri) > S.edge_thr);
+  if (EDGE && S.edge) good = good & !(hex_norm_axial(q - qi, r - ri) > S.edge_thr);
   return good ? pid : -1;
 }
 

@@ -688,8 +688,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       unsafeAtomicAdd(A.out + pix, v);
   };
   const long long total = (long long)A.nf * A.M;
-  // no sample needs the parallel-ray test (the flag after the records)
-  const bool par_free = as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u;
+  // no sample needs the parallel-ray test (the flag after the records) and
+  // the sensor has no edge test: the lean loop serves the launch
+  const bool lean =
+      as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u && !A.S.edge;
   for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
        item += (long long)gridDim.x * wpb) {
     // source chunk fastest: consecutive waves share the sample group
@@ -725,10 +727,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     }
     // one sample's rays, with shadow tests (SH) or without; the shadow
     // kernel forms the reflection from n (its records' m words stay unread:
-    // its scalar registers are the scarcer resource)
-    auto trace_one = [&](auto sh, auto par, const RecRaw& rec) {
+    // its scalar registers are the scarcer resource).  GEN (generic) false:
+    // no parallel-ray test and no edge test (the launch needs neither).
+    auto trace_one = [&](auto sh, auto gen, const RecRaw& rec) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
-      constexpr bool PAR = decltype(par)::value;
+      constexpr bool PAR = decltype(gen)::value;
       const Sample smp0 = unpack_record(rec);
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
@@ -743,7 +746,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
           // run of equal pixels per lane and source (consecutive samples of
           // a facet put a source's rays on the same one or two pixels), added
           // when it ends
-          const int pix = grid_pixel<KIND>(A.S, table, gu, gv, ok);
+          const int pix = grid_pixel<KIND, PAR>(A.S, table, gu, gv, ok);
           if (pix != run_pix[k]) {
             const float rv = value[k] * run_v[k];
             if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
@@ -789,11 +792,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       // padding at its end, so the prefetch needs no clamping)
       const float4* rp = recs + 4 * (size_t)j;
       RecRaw cur = fetch_record(rp);
-      auto run = [&](auto sh, auto par) {
+      auto run = [&](auto sh, auto gen) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
-          trace_one(sh, par, cur);
+          trace_one(sh, gen, cur);
           cur = nxt;
         }
       };
@@ -802,7 +805,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         // this kernel scalar registers)
         run(std::true_type{}, std::true_type{});
       } else {
-        if (par_free)
+        if (lean)
           run(std::false_type{}, std::false_type{});
         else
           run(std::false_type{}, std::true_type{});

@@ -570,10 +570,11 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
 
 // Hard pixel of a traced ray's hit in sensor-grid coordinates; -1 when
 // dropped or when the trace rejected the ray (ok false).
-template <int KIND>
+// EDGE false: the caller knows that the sensor has no edge test.
+template <int KIND, bool EDGE = true>
 IACT_DEV int grid_pixel(const DevSensor& S, const int32_t* table, float gu, float gv, bool ok) {
-  if (KIND == IACT_SENSOR_SQUARE) return square_pixel_hit(S, gu, gv, ok);
-  return hex_pixel_hit(S, table, gu, gv, ok);
+  if (KIND == IACT_SENSOR_SQUARE) return square_pixel_hit<EDGE>(S, gu, gv, ok);
+  return hex_pixel_hit<EDGE>(S, table, gu, gv, ok);
 }
 
 // Hard pixel of a hit (x, y) in the sensor frame.
