@@ -268,7 +268,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   if (KIND == IACT_SENSOR_HEX)
     for (int i = threadIdx.x; i < A.S.tq * A.S.tr; i += blockDim.x) tab[i] = A.S.table[i];
   __syncthreads();
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const bool want_src = !BATCH && B.g_src != nullptr, want_val = !BATCH && B.g_val != nullptr;
   const bool want_asph = !BATCH && B.g_asph != nullptr && B.surface_index != nullptr &&
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   const long long n_bitems = (n_items + wpb - 1) / wpb;
   int cur = 0;  // instance whose image gradient the LDS holds
 
-  for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += gridDim.x) {
+  for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += grid_blocks()) {
     const int inst = item_instance(A, bi * wpb);  // block-uniform
     if (BATCH && inst != cur) {
       __syncthreads();
@@ -288,7 +288,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     }
     cur = inst;
     const long long item = bi * wpb + wave;
-    WorkItem w;
+    // initialised: an item that is skipped then carries no values over from
+    // the previous one, which would join with lane-dependent control flow
+    // and make the uniform source loop look divergent to the compiler
+    WorkItem w = {0, 0, 0, 0, 0, 0};
     if (item >= n_items || !decode_item(A, item, inst, lane, w)) continue;
     const int f = w.f, m = w.m;
     const bool active = m < A.M;
@@ -311,7 +314,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     const cptr<float> vals = as_const(A.values + w.sbase);
     f3 e_lane = mk3(0.f, 0.f, 0.f);  // source offsets, 64 at a time (see k_trace)
     long long chunk = -1;            // candidate mask per source chunk (see k_trace)
-    for (int s = s0; s < s1; ++s) {
+    for (int si = s0; si < s1; ++si) {
+      // the source index as an opaque scalar: derived lane values (s + lane)
+      // cannot turn the loop's induction variable into a vector one
+      const int s = __builtin_amdgcn_readfirstlane(si);
       const int k = (s - s0) & 63;
       if (k == 0) {
         const int sl = min(s + lane, s1 - 1);

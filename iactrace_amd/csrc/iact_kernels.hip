@@ -447,7 +447,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   const bool lds_image = LEAN || A.lds_image;
   const bool det = !LEAN && A.det;
   const int acc = LEAN ? 0 : A.acc;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const int P = A.S.n_pixels;
   float* img_lds = smem;
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
   int cur = 0;  // instance whose image the LDS holds
   // the waves of a block take consecutive items, which belong to one
   // instance; everything down to `continue` is block-uniform
-  for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += gridDim.x) {
+  for (long long bi = xcd_block(A.xcd_order); bi < n_bitems; bi += grid_blocks()) {
     const int inst = item_instance(A, bi * wpb);
     if (MODE == IACT_MODE_IMAGE && lds_image && inst != cur) {
       __syncthreads();
@@ -484,7 +484,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     }
     cur = inst;
     const long long item = bi * wpb + wave;
-    WorkItem w;
+    // initialised: an item that is skipped then carries no values over from
+    // the previous one, which would join with lane-dependent control flow
+    // and make the uniform source loop look divergent to the compiler
+    WorkItem w = {0, 0, 0, 0, 0, 0};
     if (item >= n_items || !decode_item(A, item, inst, lane, w)) continue;
     const int f = w.f, m = w.m;
     const bool active = m < A.M;
@@ -664,7 +667,7 @@ __global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs
 template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL, int PART>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   extern __shared__ float smem[];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const int P = A.S.n_pixels;
   float* img = smem;
@@ -693,7 +696,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   const bool lean =
       as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u && !A.S.edge;
   for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
-       item += (long long)gridDim.x * wpb) {
+       item += (long long)grid_blocks() * wpb) {
     // source chunk fastest: consecutive waves share the sample group
     const uint32_t g = fdiv((uint32_t)item, A.d_sc);
     const int c = (int)((uint32_t)item - g * A.n_sc);
@@ -842,11 +845,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
     for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
   }
   const int32_t* table = tab_lds;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const long long total = (long long)A.nf * A.M;
   const long long nchunks = (long long)A.nf * A.cpf;
-  for (int s = blockIdx.x; s < A.ns; s += gridDim.x) {
+  for (int s = blockIdx.x; s < A.ns; s += grid_blocks()) {
     if (A.lds_image)
       for (int i = threadIdx.x; i < P; i += blockDim.x) row[i] = 0.f;
     __syncthreads();

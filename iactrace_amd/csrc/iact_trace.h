@@ -120,8 +120,16 @@ struct TraceArgs {
 // Position of a block in an XCD-major order.  Workgroups are dispatched to
 // the 8 XCDs round-robin (block b runs on XCD b % 8), so consecutive logical
 // blocks share one XCD's L2.  A permutation of [0, gridDim.x).
+// Grid size in blocks and block size in waves as values the compiler knows
+// to be uniform: the dispatch-packet loads behind gridDim / blockDim can
+// otherwise reach a loop through a join after lane-dependent loops (the LDS
+// fills at a kernel's start), which marks everything derived from them --
+// and the loops they bound -- divergent.
+IACT_DEV int grid_blocks() { return __builtin_amdgcn_readfirstlane((int)gridDim.x); }
+IACT_DEV int block_waves() { return __builtin_amdgcn_readfirstlane((int)(blockDim.x >> 6)); }
+
 IACT_DEV long long xcd_block(int on) {
-  const int G = gridDim.x, b = blockIdx.x;
+  const int G = grid_blocks(), b = blockIdx.x;
   if (!on) return b;
   const int x = b & 7, per = G >> 3, rem = G & 7;
   return (long long)x * per + min(x, rem) + (b >> 3);
