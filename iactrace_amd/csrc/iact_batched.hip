@@ -158,8 +158,31 @@ static int batch_shape(const iact_scene* scene, const iact_batch* batch, int sou
   return IACT_OK;
 }
 
+// The batched forward traces instance by instance with the source-major
+// kernels (iact_trace_fwd on the instance's rotations and sources) wherever
+// a plain render would take that layout -- primary-only scenes, not
+// deterministic, fewer than 2^31 samples -- unless the launch asks for the
+// sample-major layout.  The backward keeps the batched adjoint.
+static bool batch_source_major(const iact_scene* scene, const iact_launch* launch) {
+  if (scene->n_stages > 0) return false;
+  if (launch && (launch->layout == 1 || launch->deterministic)) return false;
+  return (long long)scene->facets.n_facets * scene->facets.n_samples < (1ll << 31);
+}
+
+// Workspace of one instance's source-major forward with ns sources: the
+// record table and the culling tables after it (an upper bound for an
+// instance that takes the sample-major layout, which has no records).
+static size_t instance_need(const iact_scene* scene, long long ns, int words, int all) {
+  const size_t recs = (size_t)IACT_REC_WORDS *
+                      ((size_t)scene->facets.n_facets * scene->facets.n_samples + 2) *
+                      sizeof(float);
+  const CullSpec c{ns, 1, nullptr, true};
+  return cull_layout(scene->facets.n_facets, words, all, recs, c).end;
+}
+
 // Workspace of a batch: prefix and instance table, then the candidate
-// tables when the batch culls (known without a device).
+// tables when the batch culls (known without a device); at least one
+// instance's source-major forward when the forward takes that path.
 static int batch_need(const iact_scene* scene, const iact_batch* batch, const iact_launch* launch,
                       size_t& need) {
   int rc = validate_scene(scene);
@@ -171,9 +194,12 @@ static int batch_need(const iact_scene* scene, const iact_batch* batch, const ia
   int words = 0, all = 0;
   rc = cull_words_for(*scene, launch, words, all);
   if (rc) return rc;
+  need = batch_source_major(scene, launch) ? instance_need(scene, max_ns, words, all) : 0;
   if (total >= (1ll << 31)) all = 1;  // as batch_shape
   const CullSpec c{total, batch->n_instances, batch->source_offsets, true};
-  need = cull_layout(scene->facets.n_facets, words, all, batch_ws_bytes(batch->n_instances), c).end;
+  const size_t bat =
+      cull_layout(scene->facets.n_facets, words, all, batch_ws_bytes(batch->n_instances), c).end;
+  if (bat > need) need = bat;
   return IACT_OK;
 }
 
@@ -259,6 +285,31 @@ size_t iact_batch_workspace_size(const iact_scene* scene, const iact_batch* batc
 int32_t iact_trace_batched_fwd(const iact_scene* scene, const iact_batch* batch,
                                int32_t source_type, const iact_launch* launch, void* workspace,
                                size_t workspace_bytes, float* images, void* stream) {
+  if (scene && batch && batch_source_major(scene, launch)) {
+    // instance by instance, sharing the workspace (one stream orders them);
+    // checks in the order of the batched kernel's path
+    if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL)
+      return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
+    size_t need = 0;
+    int rc = batch_need(scene, batch, launch, need);  // validates scene and batch
+    if (rc) return rc;
+    if (!workspace || workspace_bytes < need)
+      return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                       workspace_bytes);
+    if (!images) return set_error(IACT_ERR_INVALID, "null images");
+    const long long P = scene->sensor.n_pixels;
+    const int64_t* h = batch->host_source_offsets;
+    for (int i = 0; i < batch->n_instances; ++i) {
+      iact_scene si = *scene;
+      si.facets.rotations = batch->rotations + (size_t)i * 3 * scene->facets.n_facets;
+      rc = iact_trace_fwd(&si, batch->sources + 3 * h[i], batch->values + h[i],
+                          (int32_t)(h[i + 1] - h[i]), source_type, IACT_MODE_IMAGE, launch,
+                          workspace, workspace_bytes, images + (size_t)i * P, nullptr, nullptr,
+                          stream);
+      if (rc) return rc;
+    }
+    return IACT_OK;
+  }
   TraceArgs A;
   LaunchShape shp;
   bool empty;

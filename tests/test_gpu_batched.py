@@ -47,14 +47,16 @@ def _small_c3(n_inst=5, sizes=(0, 7, 300, 1, 64)):
     return tel, rot.contiguous(), s, v, off
 
 
-@pytest.mark.parametrize("tile", [0, 16])
-def test_batched_forward_matches_plain_hex_obstructions(tile):
+@pytest.mark.parametrize("tile,layout", [(0, 0), (0, 1), (16, 1)])
+def test_batched_forward_matches_plain_hex_obstructions(tile, layout):
+    """layout 0: instance by instance with the plain render's kernels;
+    layout 1: the sample-major kernel over the whole batch's work list."""
     tel, rot, s, v, off = _small_c3()
-    launch = ia.LaunchConfig(source_tile=tile)
+    launch = ia.LaunchConfig(source_tile=tile, layout=layout)
     img_b = ia.render_batched(tel, rot, s, v, off, "point", launch=launch)
     img_p = _instances(tel, rot, s, v, off, "point")
-    print(f"[batched-hex tile={tile}] instances {len(off) - 1}, sources {np.diff(off).tolist()}",
-          flush=True)
+    print(f"[batched-hex tile={tile} layout={layout}] instances {len(off) - 1}, "
+          f"sources {np.diff(off).tolist()}", flush=True)
     assert img_b.shape == img_p.shape
     for i in range(len(off) - 1):
         a, b = img_b[i].cpu().numpy(), img_p[i].cpu().numpy()
