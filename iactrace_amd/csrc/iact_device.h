@@ -537,9 +537,10 @@ IACT_DEV GridK grid_consts(const DevSensor& S, const PlaneK& K) {
 }
 
 // plane_xy in grid coordinates (gu, gv); the same miss conditions.
-// PAR false leaves out the parallel-ray test, for origins where a ray that
-// fails it lands off the grid anyway (plane_test_redundant).
-template <int KIND, bool AXIS, bool PAR = true>
+// GEN (generic) false: the launch needs no parallel-ray test (a ray that
+// fails it lands off the grid anyway, plane_test_redundant) and the pixels
+// are square (v_scale = 1).
+template <int KIND, bool AXIS, bool GEN = true>
 IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, float& gv,
                          float rs = 1.f) {
   float nd, ru, rv;
@@ -555,12 +556,12 @@ IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, fl
   const float t = G.kt * frcp(nd);
   if (KIND == IACT_SENSOR_SQUARE) {
     gu = fmaf(t, ru, G.ku);
-    gv = fmaf(t, rv * S.v_scale, G.kv);
+    gv = fmaf(t, GEN ? rv * S.v_scale : rv, G.kv);
   } else {
     gu = fmaf(t, fmaf(ru, S.aqx, rv * S.aqy), G.ku);
     gv = fmaf(t, fmaf(ru, S.arx, rv * S.ary), G.kv);
   }
-  return (!PAR || !(fabsf(nd * rs) < 1e-10f)) && t > 0.f;
+  return (!GEN || !(fabsf(nd * rs) < 1e-10f)) && t > 0.f;
 }
 
 // True when the parallel-ray test (|r_n| < 1e-10 for a unit r) cannot change

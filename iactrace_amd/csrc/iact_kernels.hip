@@ -691,10 +691,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       unsafeAtomicAdd(A.out + pix, v);
   };
   const long long total = (long long)A.nf * A.M;
-  // no sample needs the parallel-ray test (the flag after the records) and
-  // the sensor has no edge test: the lean loop serves the launch
-  const bool lean =
-      as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u && !A.S.edge;
+  // no sample needs the parallel-ray test (the flag after the records), the
+  // sensor has no edge test and square pixels: the lean loop serves the launch
+  const bool lean = as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u &&
+                    !A.S.edge && (KIND != IACT_SENSOR_SQUARE || A.S.v_scale == 1.f);
   for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
        item += (long long)grid_blocks() * wpb) {
     // source chunk fastest: consecutive waves share the sample group
@@ -731,10 +731,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     // one sample's rays, with shadow tests (SH) or without; the shadow
     // kernel forms the reflection from n (its records' m words stay unread:
     // its scalar registers are the scarcer resource).  GEN (generic) false:
-    // no parallel-ray test and no edge test (the launch needs neither).
+    // no parallel-ray test, no edge test and square pixels (the launch needs
+    // neither test and has such pixels).
     auto trace_one = [&](auto sh, auto gen, const RecRaw& rec) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
-      constexpr bool PAR = decltype(gen)::value;
+      constexpr bool GEN = decltype(gen)::value;
       const Sample smp0 = unpack_record(rec);
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
@@ -743,13 +744,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
-        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0, PAR>(
+        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0, GEN>(
             A, smp, true, e[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
         if (!SPLAT) {
           // run of equal pixels per lane and source (consecutive samples of
           // a facet put a source's rays on the same one or two pixels), added
           // when it ends
-          const int pix = grid_pixel<KIND, PAR>(A.S, table, gu, gv, ok);
+          const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
           if (pix != run_pix[k]) {
             const float rv = value[k] * run_v[k];
             if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);

@@ -519,8 +519,9 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 }
 
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
-// PAR false (image traces of samples whose plane_test_redundant holds) leaves
-// out the parallel-ray test; ok may then hold for a ray that lands far off
+// GEN false (image traces of launches whose samples all satisfy
+// plane_test_redundant, on square pixels) leaves out the parallel-ray test
+// and the pixel aspect ratio; ok may then hold for a ray that lands far off
 // the grid, which binning drops.
 // e is the facet's source offset (source_offset).  Returns false when the
 // ray misses the plane (the reference's 1e10 sentinel) or the lane is
@@ -528,7 +529,7 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 // coordinates (what the binning reads); XY also gives the hit (x, y) in the
 // sensor frame.
 template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false,
-          bool RM = (OBST == 0), bool PAR = true>
+          bool RM = (OBST == 0), bool GEN = true>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float value,
                         float& gu, float& gv, float& v, float& x, float& y,
                         float* vfac = nullptr) {
@@ -570,7 +571,7 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
     K = plane_consts(A.S, o);
     G = grid_consts<KIND>(A.S, K);
   }
-  const bool ok = plane_grid<KIND, AXIS, PAR || STAGES>(A.S, G, r, gu, gv, rs);
+  const bool ok = plane_grid<KIND, AXIS, GEN || STAGES>(A.S, G, r, gu, gv, rs);
   if (XY) plane_xy<AXIS>(A.S, K, r, x, y, rs);
   if (!active) v = 0.f;
   return ok && active;

@@ -106,6 +106,20 @@ def test_layouts_c3_hex_obstructions(layout):
 
 
 @pytest.mark.parametrize("layout", [1, 2])
+def test_non_square_pixels(layout):
+    """Pixels 0.8 times as tall as wide: the source-major trace takes its
+    generic loop (the aspect ratio enters the hit's row coordinate)."""
+    tel, wl = _tel("C2", ns=80, m=16, key=9)
+    s = tel.sensors[0]
+    b = [float(v) for v in s.bounds]
+    tel = tel.replace_sensor(ia.SquareSensor(s.position, s.rotation, s.width, s.height,
+                                             (b[0], b[1], 0.8 * b[2], 0.8 * b[3])), 0)
+    img, _ = _trace_and_compare(tel, wl.sources, wl.values, "point", f"aspect 0.8 {layout}",
+                                launch=ia.LaunchConfig(layout=layout))
+    assert img.sum() > 0
+
+
+@pytest.mark.parametrize("layout", [1, 2])
 def test_facet_in_the_sensor_plane(layout):
     """A flat, untilted facet lying in the sensor plane (its samples are at
     distance 0 from it): the source-major trace then keeps the parallel-ray
