@@ -643,6 +643,19 @@ IACT_DEV void hex_round(float q, float r, float& qi_out, float& ri_out) {
   ri_out = fix_r ? -qi - si : ri;
 }
 
+// Nearest hex centre by rounding q and r and moving at most one step along
+// the larger remainder (J. Rus's axial rounding): the cell of hex_round
+// except for points within rounding of a cell boundary, in 13 instead of 15
+// vector instructions.
+IACT_DEV void hex_round_nearest(float q, float r, float& qi_out, float& ri_out) {
+  const float qg = rintf(q), rg = rintf(r);
+  const float xr = q - qg, yr = r - rg;
+  const bool c = fabsf(xr) >= fabsf(yr);
+  const float t = rintf(c ? fmaf(0.5f, yr, xr) : fmaf(0.5f, xr, yr));
+  qi_out = c ? qg + t : qg;
+  ri_out = c ? rg : rg + t;
+}
+
 // Table lookup with bounds (hexagonal.py:155-172); -1 when off-table/empty.
 IACT_DEV int hex_lookup(const DevSensor& S, const int32_t* table, float qi, float ri) {
   const float qa = qi - S.qmin_f, ra = ri - S.rmin_f;
@@ -678,7 +691,8 @@ IACT_DEV int trunc_i32(float x) {
 
 // Hard hex binning of a traced ray's hit (table-relative fractional axial
 // q, r), -1 when the trace rejected the ray (ok false): the pixel of
-// hex_pixel_grid, with the range test as two unsigned compares, the table
+// hex_pixel_grid (up to points within rounding of a cell boundary, see
+// hex_round_nearest), with the range test as two unsigned compares, the table
 // read for every lane (the index clamped into the table) and then selected,
 // and the edge test only for sensors with an edge.  Without one
 // (edge_thr = 1) the reference's test can only drop a point whose hex norm
@@ -686,7 +700,7 @@ IACT_DEV int trunc_i32(float x) {
 template <bool EDGE = true>
 IACT_DEV int hex_pixel_hit(const DevSensor& S, const int32_t* table, float q, float r, bool ok) {
   float qi, ri;
-  hex_round(q, r, qi, ri);
+  hex_round_nearest(q, r, qi, ri);
   const int iq = trunc_i32(qi), ir = trunc_i32(ri);
   const bool in = ((unsigned)iq < (unsigned)S.tq) & ((unsigned)ir < (unsigned)S.tr);
   const unsigned last = __umul24((unsigned)S.tq, (unsigned)S.tr) - 1u;
