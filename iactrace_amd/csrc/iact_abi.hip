@@ -443,8 +443,15 @@ CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& 
   o = align256(o + (size_t)nf * words * sizeof(uint32_t));
   L.chunks = !all && c.n_src >= kChunkCullMinSources && c.n_src < (1ll << 31);
   if (L.chunks) {
+    const size_t nch = (size_t)((c.n_src + 63) / 64);
     L.cmasks = o;
-    o = align256(o + (size_t)((c.n_src + 63) / 64) * nf * words * sizeof(uint32_t));
+    o = align256(o + nch * nf * words * sizeof(uint32_t));
+    L.ccones = o;
+    o = align256(o + nch * nf * 4 * sizeof(float));
+    if (c.n_groups > 0) {
+      L.gbounds = o;
+      o = align256(o + (size_t)c.n_groups * 4 * sizeof(float));
+    }
   }
   L.sort = L.chunks && c.sort;
   if (L.sort) {
@@ -464,7 +471,7 @@ CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& 
 size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
                            const TraceArgs& A) {
   const size_t base = base_workspace_bytes(sc, mode, shp, A);
-  const CullSpec c{ns, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det};
+  const CullSpec c{ns, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det, refine_groups(A, mode)};
   return cull_layout(A.nf, A.cull_words, A.cull_all, base, c).end;
 }
 
@@ -497,12 +504,21 @@ int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hi
   A.cull = fmasks;
   if (L.chunks) {
     uint32_t* cm = (uint32_t*)(ws + L.cmasks);
-    e = launch_chunk_masks(C, source_type, c.n_src, cones, fmasks, cm, st);
+    float* cc = (float*)(ws + L.ccones);
+    e = launch_chunk_masks(C, source_type, c.n_src, cones, fmasks, cm, cc, st);
     if (e != hipSuccess) return hip_error(e, "k_chunk_masks");
     A.cull = cm;
     A.cull_chunks = 1;
+    A.ccones = cc;
+    // filled by the caller once the sample records exist
+    if (c.n_groups > 0) A.gbounds = (const float*)(ws + L.gbounds);
   }
   return IACT_OK;
+}
+
+long long refine_groups(const TraceArgs& A, int mode) {
+  if (mode != IACT_MODE_IMAGE || A.layout != 2 || !has_obstructions(A.ob)) return 0;
+  return ((long long)A.nf * A.M + 63) / 64;
 }
 
 }  // namespace iact
@@ -663,7 +679,8 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
   hipError_t e = hipSuccess;
   const bool empty = (sc.facets.n_facets == 0 || n_sources == 0);
   if (!empty) {
-    const CullSpec c{n_sources, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det};
+    const CullSpec c{n_sources, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det,
+                     refine_groups(A, mode)};
     rc = prepare_cull(A, workspace, base_workspace_bytes(sc, mode, shp, A), source_type, st, c,
                       nullptr);
     if (rc) return rc;
@@ -678,6 +695,10 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
       A.recs = (const float*)workspace;
       e = launch_records(A, (float*)workspace, st);
       if (e != hipSuccess) return hip_error(e, "k_sample_records");
+      if (A.gbounds) {
+        e = launch_group_bounds(A, (float*)A.gbounds, st);
+        if (e != hipSuccess) return hip_error(e, "k_group_bounds");
+      }
       e = launch_trace_sm(A, shp, source_type, st);
       return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_sm");
     }

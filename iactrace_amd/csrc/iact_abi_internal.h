@@ -35,11 +35,17 @@ struct CullSpec {
   int n_inst;              // instances (1 for a plain trace)
   const int64_t* offsets;  // device (n_inst + 1) source prefix, or NULL
   bool sort;               // the trace may reorder its sources
+  long long n_groups = 0;  // blocks of 64 samples whose bounds refine the
+                           // chunk masks (source-major image traces), or 0
 };
 struct CullLayout {
-  size_t cones, fmasks, cmasks, sws, perm, src, val, end;  // byte offsets
+  // byte offsets: facet cones and masks; per (chunk, facet) masks and cone
+  // axes; bounds of the sample blocks; source sort
+  size_t cones, fmasks, cmasks, ccones, gbounds, sws, perm, src, val, end;
   bool chunks, sort;
 };
+// Sample blocks of a trace whose chunk masks the trace refines per block.
+long long refine_groups(const TraceArgs& A, int mode);
 int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all);
 CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& c);
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,

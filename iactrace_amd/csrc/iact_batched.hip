@@ -176,7 +176,8 @@ static size_t instance_need(const iact_scene* scene, long long ns, int words, in
   const size_t recs = (size_t)IACT_REC_WORDS *
                       ((size_t)scene->facets.n_facets * scene->facets.n_samples + 2) *
                       sizeof(float);
-  const CullSpec c{ns, 1, nullptr, true};
+  const long long total = (long long)scene->facets.n_facets * scene->facets.n_samples;
+  const CullSpec c{ns, 1, nullptr, true, (total + 63) / 64};
   return cull_layout(scene->facets.n_facets, words, all, recs, c).end;
 }
 

@@ -399,6 +399,42 @@ IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool activ
   return (blocked && active) ? 0.f : 1.f;
 }
 
+// A copy of v that the compiler cannot trace back to v: arithmetic on it
+// stays in the branch that makes the copy, where it would otherwise be
+// hoisted out of the candidate loop and run for every ray with candidates,
+// whichever obstruction kinds they are.
+IACT_DEV f3 pinned(f3 v) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
+  return v;
+}
+
+// Ray parameter of the hit of obstruction j (mask numbering: cylinders,
+// boxes, spheres, oriented boxes, triangles), +inf for a miss.
+IACT_DEV float hit_obstruction(const iact_obstructions& ob, int j, f3 o, f3 d) {
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
+  const int c3 = c2 + ob.n_oriented_boxes;
+  if (j < c0) return hit_cylinder(o, d, as_const(ob.cylinders) + 8 * j);
+  if (j < c1) return hit_box(o, pinned(d), as_const(ob.boxes) + 6 * (j - c0));
+  if (j < c2) return hit_sphere(o, pinned(d), as_const(ob.spheres) + 4 * (j - c1));
+  if (j < c3) return hit_oriented_box(o, pinned(d), as_const(ob.oriented_boxes) + 15 * (j - c2));
+  return hit_triangle(o, pinned(d), as_const(ob.triangles) + 9 * (j - c3));
+}
+
+// shadow_factor over a candidate list held one per lane: bit i of `bits`
+// selects the obstruction index in lane i of `list` (wave-uniform bits).
+IACT_DEV float shadow_factor_list(const iact_obstructions& ob, int list, uint64_t bits, f3 o,
+                                  f3 d, bool active) {
+  bool blocked = !active;
+  while (bits != 0ull) {
+    if (__all(blocked)) break;
+    const int pos = __builtin_ctzll(bits);
+    bits &= bits - 1ull;
+    const int j = __builtin_amdgcn_readlane(list, pos);
+    blocked |= hit_obstruction(ob, j, o, d) < 1e10f;
+  }
+  return (blocked && active) ? 0.f : 1.f;
+}
+
 // shadow_factor over a candidate list: bit j of mask word w selects
 // obstruction 32 w + j in the order cylinders, boxes, spheres, oriented
 // boxes, triangles.  The list is a superset of the obstructions the rays can
@@ -409,15 +445,6 @@ IACT_DEV float shadow_factor(const iact_obstructions& ob, f3 o, f3 d, bool activ
 // once, before the per-ray set-up that the compiler hoists out of the tests,
 // and without a scalar-memory wait (which would also wait for the caller's
 // prefetches).
-// A copy of v that the compiler cannot trace back to v: arithmetic on it
-// stays in the branch that makes the copy, where it would otherwise be
-// hoisted out of the candidate loop and run for every ray with candidates,
-// whichever obstruction kinds they are.
-IACT_DEV f3 pinned(f3 v) {
-  asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
-  return v;
-}
-
 IACT_DEV float shadow_factor_culled(const iact_obstructions& ob, cptr<uint32_t> mask, int words,
                                     uint32_t first, int any, f3 o, f3 d, bool active) {
   const float big = 1e10f;

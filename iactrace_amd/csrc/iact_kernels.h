@@ -44,7 +44,9 @@ hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cone
                        uint32_t* masks, hipStream_t st);
 hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_sources,
                               const float* cones, const uint32_t* facet_masks, uint32_t* out,
-                              hipStream_t st);
+                              float* chunk_cones, hipStream_t st);
+// Bounding balls of the blocks of 64 consecutive sample records (A.recs).
+hipError_t launch_group_bounds(const TraceArgs& A, float* bounds, hipStream_t st);
 // Source ordering for chunk culling (iact_sort.hip): sorts the sources of
 // each instance (offsets: n_inst + 1 device prefix, or NULL for one) into
 // direction cells; writes perm (sorted -> original), sorted sources/values.

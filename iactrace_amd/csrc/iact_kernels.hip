@@ -284,6 +284,53 @@ IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, floa
   return bundle_reaches(c, rho, a, tn, ctr, R);
 }
 
+// Can a ray of the bundle reach cylinder q (p1, unit axis, height, radius)?
+// The cylinder lies in the capsule of its axis segment, so it is necessary
+// that bundle_reaches holds for some axis point b(t) = p1 + t axis, t in
+// [0, h], with R = the radius: along the segment the axial coordinate s(t)
+// and the bound g(t) = rr + tan(theta) (s(t) + rr) are linear and the radial
+// distance squared e(t)^2 quadratic, so the test is whether
+// Q(t) = e(t)^2 - g(t)^2 <= 0 somewhere on the interval where s(t) >= -rr
+// and g(t) >= 0 (ends and the vertex of Q).  Q gets a tolerance of 1e-5 of
+// its terms' magnitude; NaN geometry keeps the candidate.
+IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn) {
+  const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
+  const float h = q[6], r = q[7];
+  const float rr = fmaf(r, 1.001f, 1e-3f) + rho;
+  const f3 w0 = p1 - c;
+  const float s0 = dot(w0, a), s1 = dot(ax, a);
+  const f3 P0 = fma3(-s0, a, w0), P1 = fma3(-s1, a, ax);
+  const float G0 = fmaf(tn, s0 + rr, rr), G1 = tn * s1;
+  // interval of t with s(t) + rr >= 0 and g(t) >= 0
+  float lo = 0.f, hi = h;
+  auto clip = [&](float k0, float k1) {  // k0 + k1 t >= 0
+    if (k1 > 0.f) lo = fmaxf(lo, -k0 / k1);
+    else if (k1 < 0.f) hi = fminf(hi, -k0 / k1);
+    else if (k0 < 0.f) hi = -1.f;
+  };
+  clip(s0 + rr, s1);
+  clip(G0, G1);
+  const float qa = dot(P1, P1) - G1 * G1;
+  const float qb = 2.f * (dot(P0, P1) - G0 * G1);
+  const float qc = dot(P0, P0) - G0 * G0;
+  auto Q = [&](float t) { return fmaf(fmaf(qa, t, qb), t, qc); };
+  float m = fminf(Q(lo), Q(hi));
+  if (qa > 0.f) m = fminf(m, Q(fminf(fmaxf(-qb / (2.f * qa), lo), hi)));
+  const float hh = fmaxf(fabsf(lo), fabsf(hi));
+  const float tol =
+      1e-5f * (dot(P0, P0) + G0 * G0 + hh * hh * (dot(P1, P1) + G1 * G1) + 1e-6f);
+  if (!(lo <= hi)) return lo != lo || hi != hi;  // empty interval (NaN: keep)
+  return !(m > tol);
+}
+
+// As obstruction_reaches, with cylinders tested as capsules (one test
+// instead of a chain of up to 64 piece spheres, and tighter).
+IACT_DEV bool obstruction_reaches_fine(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
+                                       float tn) {
+  if (j < ob.n_cylinders) return cylinder_reaches(ob.cylinders + 8 * j, c, rho, a, tn);
+  return obstruction_reaches(ob, j, c, rho, a, tn);
+}
+
 // Candidate bits: one thread per (facet, mask word).
 __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, int all) {
   const iact_obstructions& ob = A.ob;
@@ -312,7 +359,7 @@ __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, i
 // masks_out[(chunk * nf + f) * words + w].
 __global__ void k_chunk_masks(TraceArgs A, int source_type, long long n_sources,
                               const float* cones, const uint32_t* facet_masks,
-                              uint32_t* masks_out) {
+                              uint32_t* masks_out, float* chunk_cones) {
   const iact_obstructions& ob = A.ob;
   const int words = A.cull_words;
   const long long n_chunks = (n_sources + 63) / 64;
@@ -363,6 +410,11 @@ __global__ void k_chunk_masks(TraceArgs A, int source_type, long long n_sources,
     }
     const float th = ang + 2e-4f;
     const float tn = th < 1.4f ? tanf(th) : -1.f;
+    float* cc = chunk_cones + 4 * (size_t)id;
+    cc[0] = a.x;
+    cc[1] = a.y;
+    cc[2] = a.z;
+    cc[3] = tn;
     const uint32_t* fm = facet_masks + (size_t)f * words;
     uint32_t* out = masks_out + (size_t)id * words;
     for (int w = 0; w < words; ++w) {
@@ -648,6 +700,44 @@ __global__ __launch_bounds__(256) void k_sample_records(TraceArgs A, float* recs
   }
 }
 
+// Candidates of the SPL chunks (c SPL + k) of facet f for the rays of sample
+// block `blk`: each chunk's list is spread one obstruction per lane and each
+// lane re-tests its obstruction against the bundle of the block's ball and
+// the chunk's cone; rbits[k] selects the lanes of rlist[k] that may block.
+// False (no refinement) when a chunk has more than 64 candidates.
+template <int SPL>
+IACT_DEV bool refine_candidates(const TraceArgs& A, int f, int c, int blk, int lane,
+                                const FacetCull* fc, int* rlist, uint64_t* rbits) {
+  const cptr<float> gb = as_const(A.gbounds) + 4 * (size_t)blk;  // scalar loads
+  const f3 gc = mk3(gb[0], gb[1], gb[2]);
+  const float grho = gb[3];
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) {
+    int n = 0, list = 0;
+    if (fc[k].any) {
+      const cptr<uint32_t> m = as_const(A.cull) + fc[k].row;
+      for (int w = 0; w < A.cull_words; ++w) {
+        uint32_t bits = w == 0 ? fc[k].first : m[w];
+        while (bits != 0u) {
+          if (n == 64) return false;
+          list = lane == n ? 32 * w + __builtin_ctz(bits) : list;
+          bits &= bits - 1u;
+          ++n;
+        }
+      }
+    }
+    const long long chunk = min(c * SPL + k, (A.ns - 1) >> 6);
+    const cptr<float> cc = as_const(A.ccones) + 4 * (size_t)(chunk * A.nf + f);
+    const f3 ax = mk3(cc[0], cc[1], cc[2]);
+    const float tn = cc[3];
+    const bool keep =
+        lane < n && (tn < 0.f || obstruction_reaches_fine(A.ob, list, gc, grho, ax, tn));
+    rlist[k] = list;
+    rbits[k] = __ballot(keep);
+  }
+  return true;
+}
+
 // Image trace, source-major layout: lane = SPL sources (64 apart), a work
 // item is a group of `sample_group` consecutive samples (facet-major) traced
 // against a chunk of 64 SPL sources; the sample's world-space record is
@@ -719,10 +809,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const float4* recs = (const float4*)A.recs;
     int run_pix[SPL];
     float run_v[SPL];
-    // the current facet's obstruction candidates (per chunk of 64 sources)
-    // and each lane source's offset from it
+    // the current facet's obstruction candidates (per chunk of 64 sources),
+    // their refinement for the sample block, and each lane source's offset
     FacetCull fc[SPL];
     f3 e[SPL];
+    bool refined = false;
+    int rlist[SPL];
+    uint64_t rbits[SPL];
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
       run_pix[k] = -1;
@@ -741,6 +834,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       for (int k = 0; k < SPL; ++k) {
         Sample smp = smp0;
         set_facet(smp, fc[k]);
+        if (OB) {
+          smp.rmode = refined;
+          smp.rlist = rlist[k];
+          smp.rbits = rbits[k];
+        }
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
@@ -788,6 +886,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         j = jend;
         continue;
       }
+      // the shadow kernel re-tests the candidates of a run inside one block
+      // of 64 samples against the block's bundle (the block's ball, the
+      // chunk's cone), one candidate per lane; lists of more than 64 are not
+      // refined
+      refined = false;
+      if (OBST && PART == 2 && A.gbounds && (j >> 6) == ((jend - 1) >> 6))
+        refined = refine_candidates<SPL>(A, f, c, j >> 6, lane, fc, rlist, rbits);
       const cptr<float> fp = as_const(A.positions) + 3 * f;  // scalar loads
       const f3 pos = mk3(fp[0], fp[1], fp[2]);
 #pragma unroll
@@ -1047,10 +1152,46 @@ hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cone
 
 hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_sources,
                               const float* cones, const uint32_t* facet_masks, uint32_t* out,
-                              hipStream_t st) {
+                              float* chunk_cones, hipStream_t st) {
   const long long total = (n_sources + 63) / 64 * A.nf;
   hipLaunchKernelGGL(k_chunk_masks, dim3(grid_for(total, 256, 16384)), dim3(256), 0, st, A,
-                     source_type, n_sources, cones, facet_masks, out);
+                     source_type, n_sources, cones, facet_masks, out, chunk_cones);
+  return hipGetLastError();
+}
+
+// One thread per block of 64 consecutive records: the centre of the block's
+// bounding box of sample positions and the largest distance from it.
+__global__ void k_group_bounds(const float* recs, long long total, float* bounds) {
+  const long long nb = (total + 63) / 64;
+  for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
+       b += (long long)gridDim.x * blockDim.x) {
+    const long long i0 = 64 * b, i1 = min(i0 + 64, total);
+    f3 lo = mk3(3e38f, 3e38f, 3e38f), hi = mk3(-3e38f, -3e38f, -3e38f);
+    for (long long i = i0; i < i1; ++i) {
+      const float* p = recs + IACT_REC_WORDS * i + 13;
+      lo = mk3(fminf(lo.x, p[0]), fminf(lo.y, p[1]), fminf(lo.z, p[2]));
+      hi = mk3(fmaxf(hi.x, p[0]), fmaxf(hi.y, p[1]), fmaxf(hi.z, p[2]));
+    }
+    const f3 c = 0.5f * (lo + hi);
+    float r2 = 0.f;
+    for (long long i = i0; i < i1; ++i) {
+      const float* p = recs + IACT_REC_WORDS * i + 13;
+      const f3 dv = mk3(p[0], p[1], p[2]) - c;
+      r2 = fmaxf(r2, dot(dv, dv));
+    }
+    // (1e-6 relative + 1 um of slack for the rounding of centre and radius)
+    float* o = bounds + 4 * b;
+    o[0] = c.x;
+    o[1] = c.y;
+    o[2] = c.z;
+    o[3] = fmaf(sqrtf(r2), 1.000001f, 1e-6f);
+  }
+}
+
+hipError_t launch_group_bounds(const TraceArgs& A, float* bounds, hipStream_t st) {
+  const long long total = (long long)A.nf * A.M;
+  hipLaunchKernelGGL(k_group_bounds, dim3(grid_for((total + 63) / 64, 256, 4096)), dim3(256), 0,
+                     st, A.recs, total, bounds);
   return hipGetLastError();
 }
 

@@ -109,6 +109,12 @@ struct TraceArgs {
   int cull_words;
   int cull_all;                  // masks with every obstruction (culling off)
   int cull_chunks;               // masks per (source chunk of 64, facet)
+  // with chunk masks: per (chunk, facet) the axis and tan(half angle) of the
+  // chunk's ray bundle (tan < 0: not culled); for source-major image traces
+  // the bounding ball (centre, radius) of each block of 64 sample records,
+  // with which a facet run inside one block refines its chunk's candidates
+  const float* ccones;
+  const float* gbounds;
 };
 
 // World-space sample record of the source-major trace (16 words, read with
@@ -238,6 +244,11 @@ struct Sample {
   uint32_t cand0;  // the facet's first candidate-mask word
   int cand_any;    // the facet has an obstruction candidate
   size_t cand_row; // offset of the facet's candidate mask
+  // refined candidates (source-major traces): rmode set, the candidates are
+  // the lanes of rlist that rbits selects (shadow_factor_list)
+  int rmode;
+  int rlist;
+  uint64_t rbits;
 };
 
 // Offset of mask row `row` (facet, or chunk * nf + facet), wave-uniform.
@@ -291,6 +302,9 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_po
   s.cand0 = 0u;
   s.cand_any = 0;
   s.cand_row = 0;
+  s.rmode = 0;
+  s.rlist = 0;
+  s.rbits = 0ull;
   return s;
 }
 
@@ -373,6 +387,9 @@ IACT_DEV Sample unpack_record(const RecRaw& q) {
   s.cand0 = 0u;
   s.cand_any = 0;
   s.cand_row = 0;
+  s.rmode = 0;
+  s.rlist = 0;
+  s.rbits = 0ull;
   return s;
 }
 
@@ -514,6 +531,7 @@ IACT_DEV void reflect_at_stage(const TraceArgs& A, int stage, f3& o, f3& d, floa
 template <int OBST>
 IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool active) {
   if (!(OBST == 1 || (OBST == 2 && has_obstructions(A.ob)))) return 1.f;
+  if (s.rmode) return shadow_factor_list(A.ob, s.rlist, s.rbits, s.p, -d, active);
   return shadow_factor_culled(A.ob, as_const(A.cull) + s.cand_row, A.cull_words, s.cand0,
                               s.cand_any, s.p, -d, active);
 }
