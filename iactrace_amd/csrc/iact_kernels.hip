@@ -239,27 +239,15 @@ IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
 }
 
 // Can a ray of the bundle reach obstruction j (mask numbering: cylinders,
-// boxes, spheres, oriented boxes, triangles)?  Cylinders are tested as a
-// chain of bounding spheres of pieces as long as the radius (at most 64
-// pieces): a piece's sphere is then 1.12 radii across where a 1 m piece of a
-// 0.1 m mast would be 5.1, so the masks keep far fewer facets per mast.
+// boxes, spheres, oriented boxes, triangles)?  Cylinders are tested as the
+// capsule of their axis segment (cylinder_reaches), the other kinds through
+// a bounding sphere.
+IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn);
 IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
                                   float tn) {
   const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
   const int c3 = c2 + ob.n_oriented_boxes;
-  if (j < c0) {
-    const float* q = ob.cylinders + 8 * j;
-    const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
-    const float h = q[6], r = q[7];
-    // (NaN or zero radii give one piece)
-    const float per = h / fmaxf(r, 1e-6f);
-    const int n = per > 1.f ? (int)fminf(ceilf(per), 64.f) : 1;
-    const float hp = h / (float)n;
-    const float R = sqrtf(0.25f * hp * hp + r * r);
-    for (int i = 0; i < n; ++i)
-      if (bundle_reaches(c, rho, a, tn, fma3(hp * ((float)i + 0.5f), ax, p1), R)) return true;
-    return false;
-  }
+  if (j < c0) return cylinder_reaches(ob.cylinders + 8 * j, c, rho, a, tn);
   if (j < c1) {
     const float* q = ob.boxes + 6 * (j - c0);
     const f3 lo = mk3(q[0], q[1], q[2]), hi = mk3(q[3], q[4], q[5]);
@@ -323,14 +311,6 @@ IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn) 
   return !(m > tol);
 }
 
-// As obstruction_reaches, with cylinders tested as capsules (one test
-// instead of a chain of up to 64 piece spheres, and tighter).
-IACT_DEV bool obstruction_reaches_fine(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
-                                       float tn) {
-  if (j < ob.n_cylinders) return cylinder_reaches(ob.cylinders + 8 * j, c, rho, a, tn);
-  return obstruction_reaches(ob, j, c, rho, a, tn);
-}
-
 // Candidate bits: one thread per (facet, mask word).
 __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, int all) {
   const iact_obstructions& ob = A.ob;
@@ -346,7 +326,8 @@ __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, i
     const f3 a = all ? mk3(0.f, 0.f, 1.f) : mk3(cn[4], cn[5], cn[6]);
     uint32_t bits = 0u;
     for (int j = 32 * w; j < min(32 * w + 32, n_ob); ++j)
-      if (all || tn < 0.f || obstruction_reaches(ob, j, c, rho, a, tn)) bits |= 1u << (j - 32 * w);
+      if (all || tn < 0.f || obstruction_reaches(ob, j, c, rho, a, tn))
+        bits |= 1u << (j - 32 * w);
     masks[id] = bits;
   }
 }
@@ -731,7 +712,7 @@ IACT_DEV bool refine_candidates(const TraceArgs& A, int f, int c, int blk, int l
     const f3 ax = mk3(cc[0], cc[1], cc[2]);
     const float tn = cc[3];
     const bool keep =
-        lane < n && (tn < 0.f || obstruction_reaches_fine(A.ob, list, gc, grho, ax, tn));
+        lane < n && (tn < 0.f || obstruction_reaches(A.ob, list, gc, grho, ax, tn));
     rlist[k] = list;
     rbits[k] = __ballot(keep);
   }
