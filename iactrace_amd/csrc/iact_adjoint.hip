@@ -195,7 +195,7 @@ IACT_DEV void wave_add_keyed(double* table, int key, double v) {
 // and the normal m / |m|, whose change with (x, y) needs the sag's Hessian.
 // Gradients of the hit surface's aspheric coefficients go to g_asph.
 IACT_DEV void stage_adjoint(const AdjArgs& B, const StageRec& rec, f3& go, f3& gr, float& gv,
-                            bool want_asph) {
+                            bool want_asph, double* asph_table) {
   const TraceArgs& A = B.T;
   const f3 zero = mk3(0.f, 0.f, 0.f);
   const bool live = rec.mi >= 0 && rec.hs != 0.f;
@@ -242,7 +242,7 @@ IACT_DEV void stage_adjoint(const AdjArgs& B, const StageRec& rec, f3& go, f3& g
       const float dS = pw - pw0;                      // d S / d a_i
       const float dg = (float)(2 * (i + 1)) * D.r2 * q;  // d (dz/dr2) / d a_i
       const float ga = fmaf(gS + k, dS, 2.f * dg * (gSx * D.X + gSy * D.Y));
-      wave_add_keyed(B.g_asph, use ? IACT_MAX_ASPHERIC * si + i : -1, (double)ga);
+      wave_add_keyed(asph_table, use ? IACT_MAX_ASPHERIC * si + i : -1, (double)ga);
       pw *= r4;
       pw0 *= r40;
       q *= r4;
@@ -274,6 +274,15 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   const bool want_asph = !BATCH && B.g_asph != nullptr && B.surface_index != nullptr &&
                          B.offsets != nullptr;
   const bool want_stage_asph = STAGES && B.g_asph != nullptr;
+  // stage mirrors' aspheric gradients: summed per block in LDS (every wave
+  // of every block adds to the same few coefficients, which as global
+  // atomics serialise) and added to g_asph once per block at the end
+  __shared__ double s_gasph[STAGES ? IACT_MAX_SURFACES * IACT_MAX_ASPHERIC : 1];
+  if (want_stage_asph) {
+    for (int i = threadIdx.x; i < IACT_MAX_SURFACES * IACT_MAX_ASPHERIC; i += blockDim.x)
+      s_gasph[i] = 0.0;
+    __syncthreads();
+  }
   const long long n_items = total_items(A);
   const long long n_bitems = (n_items + wpb - 1) / wpb;
   int cur = 0;  // instance whose image gradient the LDS holds
@@ -379,7 +388,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       if (STAGES) {
 #pragma unroll
         for (int k = IACT_ADJ_MAX_STAGES - 1; k >= 0; --k)
-          if (k < A.n_stages) stage_adjoint(B, rec[k], go, gr, gv, want_stage_asph);
+          if (k < A.n_stages) stage_adjoint(B, rec[k], go, gr, gv, want_stage_asph, s_gasph);
       }
       // now (go, gr, gv) are the adjoints of the primary leg's origin (the
       // sample), reflected direction and value = value_s * c * nrw * shadow
@@ -510,6 +519,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       }
     }
   }
+  if (want_stage_asph) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < IACT_MAX_SURFACES * IACT_MAX_ASPHERIC; i += blockDim.x) {
+      const double v = s_gasph[i];
+      if (v != 0.0) unsafeAtomicAdd(B.g_asph + i, v);
+    }
+  }
 }
 
 using AdjKernel = void (*)(AdjArgs);
@@ -614,8 +630,12 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
     return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
                      workspace_bytes);
   const size_t lds = (size_t)sc.sensor.n_pixels * sizeof(float) + shp.lds;
-  if (lds > 65536)
-    return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS", lds);
+  // (kernels following stages >= 1 also hold the aspheric-gradient table)
+  const size_t lds_static =
+      sc.n_stages > 0 ? (size_t)IACT_MAX_SURFACES * IACT_MAX_ASPHERIC * sizeof(double) : 0;
+  if (lds + lds_static > 65536)
+    return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS",
+                     lds + lds_static);
   shp.lds = lds;
   B.grad_image = grad_image;
   B.g_rot = grads->rotations;
