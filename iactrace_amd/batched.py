@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .dlpack import as_tensor
 from .render import LaunchConfig, RenderPlan
 
 __all__ = ["pack_sources", "render_batched", "AlignmentFit", "BatchedPlan"]
@@ -62,9 +63,9 @@ class BatchedPlan:
         self.n_inst = len(self.offsets_host) - 1
         if self.n_inst < 1:
             raise ValueError("need at least one instance")
-        self.src = torch.as_tensor(sources).to(device=dev, dtype=torch.float64)
+        self.src = as_tensor(sources).to(device=dev, dtype=torch.float64)
         self.src = self.src.reshape(-1, 3).contiguous()
-        self.val = torch.as_tensor(values).to(device=dev, dtype=torch.float32)
+        self.val = as_tensor(values).to(device=dev, dtype=torch.float32)
         self.val = self.val.reshape(-1).contiguous()
         if self.src.shape[0] != self.offsets_host[-1] or self.val.shape[0] != self.src.shape[0]:
             raise ValueError("offsets do not match the packed sources/values")
@@ -138,7 +139,7 @@ def render_batched(tel, rotations, sources, values, offsets, source_type="point"
     rotations (n_instances, n_facets, 3) and per-instance sources
     sources[offsets[i]:offsets[i+1]].  Differentiable w.r.t. ``rotations``."""
     bplan = BatchedPlan(tel, sources, values, offsets, source_type, sensor_idx, launch)
-    rot = torch.as_tensor(rotations)
+    rot = as_tensor(rotations)
     if torch.is_grad_enabled() and rot.requires_grad:
         return _BatchedTrace.apply(bplan, rot)
     return bplan.forward(rot.detach().to(device=bplan.device, dtype=torch.float32).contiguous())
@@ -159,9 +160,9 @@ class AlignmentFit:
         self.bp = BatchedPlan(tel, sources, values, offsets, source_type, sensor_idx, launch)
         dev = self.bp.device
         n, nf, P = self.bp.n_inst, self.bp.nf, self.bp.n_pixels
-        self.rotations = torch.as_tensor(rotations).to(device=dev, dtype=torch.float32)
+        self.rotations = as_tensor(rotations).to(device=dev, dtype=torch.float32)
         self.rotations = self.rotations.reshape(n, nf, 3).clone().contiguous()
-        self.targets = torch.as_tensor(targets).to(device=dev, dtype=torch.float32)
+        self.targets = as_tensor(targets).to(device=dev, dtype=torch.float32)
         self.targets = self.targets.reshape(n, P).contiguous()
         self.lr = float(lr)
         self.images = torch.zeros((n, P), dtype=torch.float32, device=dev)

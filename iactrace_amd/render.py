@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .dlpack import as_tensor
 from .geometry import euler_to_matrix
 from .mirrors import AsphericDiskMirrorGroup, AsphericPolygonMirrorGroup
 from .obstructions import pack_obstruction_tables
@@ -221,9 +222,9 @@ class RenderPlan:
     def forward(self, sources, values, source_type, mode, rotations=None, positions=None):
         dev = self.device
         P = self.sensor.n_pixels
-        src = torch.as_tensor(sources).to(device=dev, dtype=torch.float64).reshape(-1, 3)
+        src = as_tensor(sources).to(device=dev, dtype=torch.float64).reshape(-1, 3)
         src = src.contiguous()
-        val = torch.as_tensor(values).to(device=dev, dtype=torch.float32).reshape(-1).contiguous()
+        val = as_tensor(values).to(device=dev, dtype=torch.float32).reshape(-1).contiguous()
         ns = src.shape[0]
         if val.shape[0] != ns:
             raise ValueError(f"{ns} sources but {val.shape[0]} values")
@@ -264,9 +265,9 @@ class RenderPlan:
     def backward(self, grad_image, sources, values, source_type, rotations, positions, need):
         """need = (sources, values, rotations, positions, aspheric) booleans."""
         dev = self.device
-        src = torch.as_tensor(sources).to(device=dev, dtype=torch.float64).reshape(-1, 3)
+        src = as_tensor(sources).to(device=dev, dtype=torch.float64).reshape(-1, 3)
         src = src.contiguous()
-        val = torch.as_tensor(values).to(device=dev, dtype=torch.float32).reshape(-1).contiguous()
+        val = as_tensor(values).to(device=dev, dtype=torch.float32).reshape(-1).contiguous()
         ns = src.shape[0]
         g_img = grad_image.detach().to(device=dev, dtype=torch.float32).reshape(-1).contiguous()
         keep = [src, val, g_img]
@@ -359,8 +360,8 @@ def render(tel, sources, values, source_type="point", sensor_idx=0, launch=None)
     """
     plan = _plan(tel, sensor_idx, launch)
     shape = plan.sensor.get_accumulator_shape()
-    sources = torch.as_tensor(sources)
-    values = torch.as_tensor(values)
+    sources = as_tensor(sources)
+    values = as_tensor(values)
     if plan.empty:
         return torch.zeros(shape, device=plan.device)
     rot = torch.cat([g.rotations for g in plan.primary], 0) if len(plan.primary) > 1 \

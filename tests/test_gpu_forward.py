@@ -324,6 +324,19 @@ def test_grazing_upward_and_onaxis_parallel_sources():
     assert np.all(xy.cpu().numpy()[up] == 1e10) and np.all(pix.cpu().numpy()[up] == -1)
 
 
+def test_dlpack_inputs():
+    """Sources and values handed over as DLPack exporters (the way CuPy or
+    JAX arrays arrive) give the image of the same tensors."""
+    from tests.test_dlpack import Exporter
+    tel, wl = _tel("C2", ns=70, m=16, key=2)
+    s = torch.as_tensor(wl.sources, device=DEV)
+    v = torch.as_tensor(wl.values, device=DEV)
+    a = ia.render(tel, s, v, "point", launch=ia.LaunchConfig(deterministic=True))
+    b = ia.render(tel, Exporter(s), Exporter(v), "point",
+                  launch=ia.LaunchConfig(deterministic=True))
+    assert a.abs().max().item() > 0 and torch.equal(a, b)
+
+
 def test_deterministic_mode_bitwise():
     tel, wl = _tel("C2", ns=64, m=64)
     L = ia.LaunchConfig(deterministic=True)
