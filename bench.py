@@ -228,6 +228,17 @@ def c5_batched(ia, configs, n_instances):
 
     wp, _ = time_steps(plain, 1, 0)
     n = bw.n_rays
+    # VALU roofline of the fit iteration, lower bound: the algorithmic flops
+    # of a ray without any obstruction test (culling skips most of them, and
+    # the tests done are not credited), the adjoint counted as 3x the forward
+    # (recompute + 2x, the convention of the kernel table)
+    from iactrace_amd.flop_model import split_per_ray
+    f32, f64 = split_per_ray("point", 0, 0, 0, 0, 0, "hex", int(bw.offsets[-1]) // n_instances)
+    fb_flops = 4.0 * (f32 + f64)
+    achieved = n * fb_flops / wg / 1e12
+    roof = {"flops_per_ray_fwd_without_obstructions": f32 + f64, "fwd_plus_bwd_factor": 4.0,
+            "achieved_tflops": achieved, "peak_tflops": PEAK_FP32 / 1e12,
+            "frac_lower_bound": achieved * 1e12 / PEAK_FP32}
     return {"workload": f"C5: {n_instances} x C3 dish (875 facets x 1024 samples, hex 2048 px, "
                         f"box + 4 cylinders), independent 0.5 mrad tilt draws, "
                         f"{int(bw.offsets[-1])} point sources in total (U{{256..16384}} per "
@@ -238,7 +249,8 @@ def c5_batched(ia, configs, n_instances):
             "fit_iteration": "HIP graph: batched fwd + L2 loss grad + batched adjoint + Adam",
             "plain_fwd_bwd_rays_per_s": n / wp, "plain_fwd_bwd_s": wp,
             "plain": f"{n_instances} x (render + autograd backward wrt tilts)",
-            "batched_vs_plain_fwd_bwd": wp / wg, "loss_at_start": loss}
+            "batched_vs_plain_fwd_bwd": wp / wg, "loss_at_start": loss,
+            "fit_iteration_valu_roofline": roof}
 
 
 def main():
