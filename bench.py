@@ -262,6 +262,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C3 forward+backward and C5 batched lines")
+    ap.add_argument("--extras", default="C3,C3T,C4,C5",
+                    help="comma list of the extra lines to run (C3 forward+backward, C3T "
+                         "culling, C4 response matrix, C5 batched)")
     ap.add_argument("--c5-instances", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (for counter calibration)")
@@ -343,10 +346,20 @@ def main():
         _sync()
         out["copy_calibration_bytes"] = {"fetch": a.numel() * 4, "write": b.numel() * 4}
     if not args.no_extras and args.config == "C2":
-        out["extras"] = {"C3_fwd_bwd": c3_fwd_bwd(ia, configs, 3, 1),
-                         "C3T_culling": c3t_culling(ia, configs, 2, 1),
-                         "C4_response": c4_response(ia, configs, 5, 1),
-                         "C5_batched": c5_batched(ia, configs, args.c5_instances)}
+        want = {e.strip() for e in args.extras.split(",") if e.strip()}
+        unknown = want - {"C3", "C3T", "C4", "C5"}
+        if unknown:
+            sys.exit(f"unknown extras: {sorted(unknown)}")
+        ex = {}
+        if "C3" in want:
+            ex["C3_fwd_bwd"] = c3_fwd_bwd(ia, configs, 3, 1)
+        if "C3T" in want:
+            ex["C3T_culling"] = c3t_culling(ia, configs, 2, 1)
+        if "C4" in want:
+            ex["C4_response"] = c4_response(ia, configs, 5, 1)
+        if "C5" in want:
+            ex["C5_batched"] = c5_batched(ia, configs, args.c5_instances)
+        out["extras"] = ex
     if not args.no_cpu:
         cb, cimg, n = cpu_baseline(wl, tel)
         out["cpu_baseline"] = cb
