@@ -188,8 +188,41 @@ def cpu_baseline(wl, tel, target_s=15.0):
             "kind": "port",
             "sample": f"{wl.name}: first {n} of {len(wl.sources)} sources x all "
                       f"{sc.points.shape[0]} facets x {sc.points.shape[1]} samples "
-                      f"({n * rays_per_src:.3g} rays, {dt:.2f} s), oracle/cpu_ref.c fp32 "
+                      f"({n * rays_per_src:.3g} rays, {dt:.2f} s), oracle/cpu_ref.cc fp32 "
                       f"with fp64 directions, gcc -O3 -march=x86-64-v3 -fopenmp"}, img, n
+
+
+def image_parity(ia, wl, tel, src, vals, band=1e-3):
+    """The GPU image of the whole workload against oracle/cpu_ref's FP64 run
+    of the same samples (every step in double, image summed in double), on
+    the host cores: relative L1 error, the largest relative error of a
+    significant pixel (>= 1e-3 of the peak) and the same after taking away
+    the pixel's edge-band mass (|value| of the FP64 rays within `band` px of
+    a binning boundary, at their pixel and the one across the nearest edge:
+    the rays an fp32 coordinate may put on either side)."""
+    import numpy as np
+    from oracle import cpu_ref
+    from tests.helpers import oracle_scene
+    sc = oracle_scene(tel)
+    keep = []
+    s = cpu_ref.scene_struct(sc, keep)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    i64, b64, st = cpu_ref.render_band(s, wl.sources, wl.values, wl.source_type,
+                                       sc.sensor.n_pixels, nthreads, 64, band)
+    dt = time.perf_counter() - t0
+    g = ia.render(tel, src, vals, wl.source_type).detach().double().cpu().numpy().reshape(-1)
+    sig = i64 > 1e-3 * i64.max()
+    d = np.abs(g - i64)
+    return {"reference": f"oracle/cpu_ref.cc FP64 run of the same samples, all {wl.n_rays:.4g} "
+                         f"rays ({dt:.1f} s on {nthreads} host threads)",
+            "l1_rel": float(d.sum() / np.abs(i64).sum()),
+            "significant_px": int(sig.sum()),
+            "max_rel_significant_px": float((d[sig] / i64[sig]).max()),
+            "max_rel_significant_px_band_resolved":
+                float((np.maximum(d - b64, 0.0)[sig] / i64[sig]).max()),
+            "band_px": band, "band_rays": st["band"], "traced_rays": st["traced"],
+            "band_fraction": st["band"] / max(st["traced"], 1)}
 
 
 def c5_batched(ia, configs, n_instances):
@@ -260,6 +293,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the image comparison with the FP64 CPU run")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C3 forward+backward and C5 batched lines")
     ap.add_argument("--extras", default="C3,C3T,C4,C5",
@@ -360,6 +395,8 @@ def main():
         if "C5" in want:
             ex["C5_batched"] = c5_batched(ia, configs, args.c5_instances)
         out["extras"] = ex
+    if not args.no_parity:
+        out["parity"] = image_parity(ia, wl, tel, src, vals)
     if not args.no_cpu:
         cb, cimg, n = cpu_baseline(wl, tel)
         out["cpu_baseline"] = cb

@@ -468,11 +468,18 @@ CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& 
   return L;
 }
 
+// The fp64 image sums of a non-deterministic image render follow the
+// culling tables (256-B aligned): n_images * n_pixels doubles.
+size_t image64_offset(size_t end) { return align256(end); }
+
 size_t workspace_bytes_for(const iact_scene& sc, int ns, int mode, const LaunchShape& shp,
                            const TraceArgs& A) {
   const size_t base = base_workspace_bytes(sc, mode, shp, A);
   const CullSpec c{ns, 1, nullptr, mode == IACT_MODE_IMAGE && !A.det, refine_groups(A, mode)};
-  return cull_layout(A.nf, A.cull_words, A.cull_all, base, c).end;
+  const size_t end = cull_layout(A.nf, A.cull_words, A.cull_all, base, c).end;
+  if (mode == IACT_MODE_IMAGE && !A.det)
+    return image64_offset(end) + (size_t)sc.sensor.n_pixels * sizeof(double);
+  return end;
 }
 
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
@@ -686,11 +693,21 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
     if (rc) return rc;
   }
   if (mode == IACT_MODE_IMAGE) {
-    if (!(A.det && A.lds_image && !empty)) {
-      e = launch_fill(out, P, 0.f, st);
-      if (e != hipSuccess) return hip_error(e, "clear image");
+    if (empty || A.det) {
+      if (!(A.det && A.lds_image && !empty)) {
+        e = launch_fill(out, P, 0.f, st);
+        if (e != hipSuccess) return hip_error(e, "clear image");
+      }
+      if (empty) return IACT_OK;
+    } else {
+      // fp64 sums, converted into `out` after the trace
+      const CullSpec c{n_sources, 1, nullptr, true, refine_groups(A, mode)};
+      const size_t end = cull_layout(A.nf, A.cull_words, A.cull_all,
+                                     base_workspace_bytes(sc, mode, shp, A), c).end;
+      A.out64 = (double*)((char*)workspace + image64_offset(end));
+      e = launch_fill_d(A.out64, P, 0.0, st);
+      if (e != hipSuccess) return hip_error(e, "clear image sums");
     }
-    if (empty) return IACT_OK;
     if (A.layout == 2) {
       A.recs = (const float*)workspace;
       e = launch_records(A, (float*)workspace, st);
@@ -700,14 +717,18 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
         if (e != hipSuccess) return hip_error(e, "k_group_bounds");
       }
       e = launch_trace_sm(A, shp, source_type, st);
-      return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace_sm");
+      if (e != hipSuccess) return hip_error(e, "k_trace_sm");
+    } else {
+      A.partials = (float*)workspace;
+      e = launch_trace(A, shp, source_type, mode, st);
+      if (e != hipSuccess) return hip_error(e, "k_trace");
     }
-    A.partials = (float*)workspace;
-    e = launch_trace(A, shp, source_type, mode, st);
-    if (e != hipSuccess) return hip_error(e, "k_trace");
     if (A.det && A.lds_image) {
       e = launch_reduce(A.partials, shp.grid, (int)P, out, st);
       if (e != hipSuccess) return hip_error(e, "k_reduce_partials");
+    } else {
+      e = launch_f64_to_f32(A.out64, out, P, st);
+      if (e != hipSuccess) return hip_error(e, "k_f64_to_f32");
     }
     return IACT_OK;
   }

@@ -47,6 +47,8 @@ struct CullLayout {
 // Sample blocks of a trace whose chunk masks the trace refines per block.
 long long refine_groups(const TraceArgs& A, int mode);
 int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all);
+// Offset of the fp64 image sums after a workspace of `end` bytes.
+size_t image64_offset(size_t end);
 CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& c);
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
                  const CullSpec& c, const int32_t** perm);

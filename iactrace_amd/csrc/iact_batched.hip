@@ -178,7 +178,18 @@ static size_t instance_need(const iact_scene* scene, long long ns, int words, in
                       sizeof(float);
   const long long total = (long long)scene->facets.n_facets * scene->facets.n_samples;
   const CullSpec c{ns, 1, nullptr, true, (total + 63) / 64};
-  return cull_layout(scene->facets.n_facets, words, all, recs, c).end;
+  const size_t end = cull_layout(scene->facets.n_facets, words, all, recs, c).end;
+  return image64_offset(end) + (size_t)scene->sensor.n_pixels * sizeof(double);
+}
+
+// Workspace end of the batched kernels' tables (prefix, instance table,
+// candidate tables); the instances' fp64 image sums follow it.
+static size_t batch_tables_end(const iact_scene* scene, const iact_batch* batch, int words,
+                               int all, long long total) {
+  if (total >= (1ll << 31)) all = 1;  // as batch_shape
+  const CullSpec c{total, batch->n_instances, batch->source_offsets, true};
+  return cull_layout(scene->facets.n_facets, words, all, batch_ws_bytes(batch->n_instances), c)
+      .end;
 }
 
 // Workspace of a batch: prefix and instance table, then the candidate
@@ -196,10 +207,8 @@ static int batch_need(const iact_scene* scene, const iact_batch* batch, const ia
   rc = cull_words_for(*scene, launch, words, all);
   if (rc) return rc;
   need = batch_source_major(scene, launch) ? instance_need(scene, max_ns, words, all) : 0;
-  if (total >= (1ll << 31)) all = 1;  // as batch_shape
-  const CullSpec c{total, batch->n_instances, batch->source_offsets, true};
-  const size_t bat =
-      cull_layout(scene->facets.n_facets, words, all, batch_ws_bytes(batch->n_instances), c).end;
+  const size_t bat = image64_offset(batch_tables_end(scene, batch, words, all, total)) +
+                     (size_t)batch->n_instances * scene->sensor.n_pixels * sizeof(double);
   if (bat > need) need = bat;
   return IACT_OK;
 }
@@ -323,13 +332,25 @@ int32_t iact_trace_batched_fwd(const iact_scene* scene, const iact_batch* batch,
   if (!images) return set_error(IACT_ERR_INVALID, "null images");
   hipStream_t st = (hipStream_t)stream;
   const long long P = scene->sensor.n_pixels;
-  hipError_t e = launch_fill(images, P * batch->n_instances, 0.f, st);
-  if (e != hipSuccess) return hip_error(e, "clear images");
-  if (empty) return IACT_OK;
+  const long long nP = P * batch->n_instances;
+  hipError_t e;
+  if (empty) {
+    e = launch_fill(images, nP, 0.f, st);
+    return e == hipSuccess ? IACT_OK : hip_error(e, "clear images");
+  }
+  int words = 0, all = 0;
+  rc = cull_words_for(*scene, launch, words, all);
+  if (rc) return rc;
+  long long total = batch->host_source_offsets[batch->n_instances];
+  A.out64 = (double*)((char*)workspace +
+                      image64_offset(batch_tables_end(scene, batch, words, all, total)));
+  if ((e = launch_fill_d(A.out64, nP, 0.0, st)) != hipSuccess) return hip_error(e, "clear sums");
   if ((e = launch_prefix(A, shp, sp, workspace, st)) != hipSuccess) return hip_error(e, "k_batch_prefix");
   A.out = images;
   e = launch_trace(A, shp, source_type, IACT_MODE_IMAGE, st);
-  return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace(batched)");
+  if (e != hipSuccess) return hip_error(e, "k_trace(batched)");
+  e = launch_f64_to_f32(A.out64, images, nP, st);
+  return e == hipSuccess ? IACT_OK : hip_error(e, "k_f64_to_f32");
 }
 
 int32_t iact_trace_batched_bwd(const iact_scene* scene, const iact_batch* batch,

@@ -780,8 +780,8 @@ IACT_DEV double wave_sum_d(double v) {
 // a pixel are summed across the wave first so the LDS (or global) image sees
 // one atomic per distinct pixel; after `max_rounds` distinct pixels the rest
 // add directly.  Must be called by the whole wave in uniform control flow.
-template <bool GLOBAL>
-IACT_DEV void wave_accumulate(float* img, int pix, float v, int max_rounds) {
+template <bool GLOBAL, class T>
+IACT_DEV void wave_accumulate(T* img, int pix, float v, int max_rounds) {
   uint64_t rem = __ballot(pix >= 0 && v != 0.f);
   int rounds = 0;
   const int lane = lane_id();
@@ -789,9 +789,9 @@ IACT_DEV void wave_accumulate(float* img, int pix, float v, int max_rounds) {
     if (rounds == max_rounds) {
       if ((rem >> lane) & 1ull) {
         if (GLOBAL)
-          unsafeAtomicAdd(img + pix, v);
+          unsafeAtomicAdd(img + pix, (T)v);
         else
-          atomicAdd(img + pix, v);
+          atomicAdd(img + pix, (T)v);
       }
       break;
     }
@@ -802,9 +802,9 @@ IACT_DEV void wave_accumulate(float* img, int pix, float v, int max_rounds) {
     const float s = wave_sum(mine ? v : 0.f);
     if (lane == leader) {
       if (GLOBAL)
-        unsafeAtomicAdd(img + key, s);
+        unsafeAtomicAdd(img + key, (T)s);
       else
-        atomicAdd(img + key, s);
+        atomicAdd(img + key, (T)s);
     }
     rem &= ~m;
     ++rounds;

@@ -56,6 +56,8 @@ hipError_t launch_sort_sources(const double* src, const float* val, long long n_
                                void* ws, int32_t* perm, double* src_out, float* val_out,
                                hipStream_t st);
 hipError_t launch_reduce(const float* partials, int nblocks, int P, float* out, hipStream_t st);
+// out[i] = (float)in[i], i < n (the fp64 image sums of a render)
+hipError_t launch_f64_to_f32(const double* in, float* out, long long n, hipStream_t st);
 
 // Arguments of the adjoint kernel (iact_adjoint.hip).
 struct AdjArgs {

@@ -21,10 +21,17 @@ __global__ void k_fill_f64(double* p, long long n, double v) {
 // Ordered sum of per-block partial images: out[i] = sum_b partials[b][i].
 __global__ void k_reduce_partials(const float* partials, int nblocks, int P, float* out) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * P + i];
-    out[i] = s;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += (double)partials[(size_t)b * P + i];
+    out[i] = (float)s;
   }
+}
+
+// fp64 image sums -> the fp32 image (round to nearest)
+__global__ void k_f64_to_f32(const double* in, float* out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
 }
 
 // ------------------------------------------------------------------ sampling
@@ -510,7 +517,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
       __syncthreads();
       for (int i = threadIdx.x; i < P; i += blockDim.x) {
         const float v = img_lds[i];
-        if (v != 0.f) unsafeAtomicAdd(A.out + (size_t)cur * P + i, v);
+        if (v != 0.f) unsafeAtomicAdd(A.out64 + (size_t)cur * P + i, (double)v);
         img_lds[i] = 0.f;
       }
       __syncthreads();
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     world_sample(A.points, A.normals, A.delta, R, pos, A.pscale[f], gi, pw, nw);
     Sample smp = make_sample<KIND>(A.S, pw, nw, A.weights[gi], pos);
     const int s0 = w.s0, s1 = w.s1;
-    float* const gout = A.out + (size_t)w.inst * P;
+    double* const gout = A.out64 + (size_t)w.inst * P;
     const double* src = A.sources + 3 * w.sbase;
     const cptr<float> vals = as_const(A.values + w.sbase);
     // the facet's source offsets, 64 sources at a time (lane k holds source
@@ -605,7 +612,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     } else {
       for (int i = threadIdx.x; i < P; i += blockDim.x) {
         const float v = img_lds[i];
-        if (v != 0.f) unsafeAtomicAdd(A.out + (size_t)cur * P + i, v);
+        if (v != 0.f) unsafeAtomicAdd(A.out64 + (size_t)cur * P + i, (double)v);
       }
     }
   }
@@ -759,7 +766,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     if (A.lds_image)
       lds_add(img + pix, v);
     else
-      unsafeAtomicAdd(A.out + pix, v);
+      unsafeAtomicAdd(A.out64 + pix, (double)v);
   };
   const long long total = (long long)A.nf * A.M;
   // no sample needs the parallel-ray test (the flag after the records), the
@@ -913,7 +920,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     __syncthreads();
     for (int i = threadIdx.x; i < P; i += blockDim.x) {
       const float v = img[i];
-      if (v != 0.f) unsafeAtomicAdd(A.out + i, v);
+      if (v != 0.f) unsafeAtomicAdd(A.out64 + i, (double)v);
     }
   }
 }
@@ -1187,6 +1194,11 @@ hipError_t launch_records(const TraceArgs& A, float* recs, hipStream_t st) {
   else
     hipLaunchKernelGGL(k_sample_records<IACT_SENSOR_HEX>, dim3(grid_for(A.nf, 1, 16384)),
                        dim3(256), 0, st, A, recs, (uint32_t*)flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_f64_to_f32(const double* in, float* out, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, in, out, n);
   return hipGetLastError();
 }
 

@@ -73,6 +73,11 @@ struct TraceArgs {
   DevSensor S;
   // outputs / scratch
   float* out;
+  // image renders (not deterministic): the image is summed here, in fp64,
+  // and converted to `out` after the trace -- an image is a sum of ~10^7
+  // rays per pixel, and fp32 atomics from thousands of blocks would round
+  // it well above 1e-5
+  double* out64;
   float* dbg_xy;
   int32_t* dbg_pix;
   float* partials;

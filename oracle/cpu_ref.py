@@ -1,4 +1,5 @@
-"""ctypes loader of oracle/libcpu_ref.so (test infrastructure / CPU baseline only)."""
+"""ctypes loader of oracle/libcpu_ref.so (test infrastructure: the CPU baseline,
+the second restatement and the FP64 run of full-size images)."""
 import ctypes
 import os
 
@@ -17,23 +18,40 @@ def lib():
         L = ctypes.CDLL(path)
         L.cpu_ref_render.restype = ctypes.c_int
         L.cpu_ref_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                     ctypes.c_void_p]
         L.cpu_ref_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
-def render(scene_struct, sources, values, source_type, n_pixels, nthreads=0):
-    """Image (n_pixels,) of a scene descriptor whose pointers are HOST arrays."""
+def render(scene_struct, sources, values, source_type, n_pixels, nthreads=0, precision=32):
+    """Image (n_pixels,) float64 of a scene descriptor whose pointers are HOST
+    arrays.  precision 32: the reference's fp32 arithmetic (FP64 source
+    directions and Newton); 64: FP64 throughout.  Accumulated in double."""
+    return render_band(scene_struct, sources, values, source_type, n_pixels, nthreads,
+                       precision, 0.0)[0]
+
+
+def render_band(scene_struct, sources, values, source_type, n_pixels, nthreads=0,
+                precision=64, band=1e-3):
+    """(image, band_image, stats): band_image[p] sums |value| of the rays whose
+    sensor-grid coordinate lies within `band` (pixels; hex-norm units) of a
+    binning boundary, at their own pixel and at the pixel across the nearest
+    cell edge; stats = {binned, band, traced} ray counts."""
     src = np.ascontiguousarray(sources, np.float64)
     val = np.ascontiguousarray(values, np.float32)
-    out = np.zeros(n_pixels, np.float32)
+    out = np.zeros(n_pixels, np.float64)
+    bimg = np.zeros(n_pixels, np.float64)
+    st3 = np.zeros(3, np.int64)
     st = 0 if source_type == "point" else 1
     rc = lib().cpu_ref_render(ctypes.byref(scene_struct), src.ctypes.data, val.ctypes.data,
-                              len(val), st, out.ctypes.data, int(nthreads))
+                              len(val), st, out.ctypes.data, int(nthreads), int(precision),
+                              float(band), bimg.ctypes.data, st3.ctypes.data)
     if rc != 0:
         raise RuntimeError("cpu_ref_render failed")
-    return out
+    return out, bimg, {"binned": int(st3[0]), "band": int(st3[1]), "traced": int(st3[2])}
 
 
 def max_threads():

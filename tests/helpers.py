@@ -139,3 +139,38 @@ def hex_kink_margin(x, y, sensor):
             bb = np.abs(dr - b)
             m = np.minimum(m, np.minimum(np.abs(e - 1.5 * bb), np.minimum(e, bb)))
     return m
+
+
+def cpu_scene(wl, key=0):
+    """The oracle Scene of a primary-only workload, sampled on the CPU with
+    the oracle's Philox sampler under the key derivation Workload.telescope
+    uses (the GPU sampler draws the same samples bit for bit), tilt errors
+    applied as Workload.telescope applies them."""
+    from iactrace_amd.geometry import PolygonAperture
+    from iactrace_amd.integrators import as_key, split_key
+    from iactrace_amd.mirrors import group_mirrors
+    from iactrace_amd.obstructions import group_obstructions
+    groups = group_mirrors(wl.mirrors)
+    assert all(g.optical_stage == 0 for g in groups), "primary-only workloads"
+    k = key
+    parts = []
+    for g in groups:
+        k, sub = split_key(k, 2)
+        surf = O.Surface(g.curvature, g.conic, g.aspheric.detach().double().numpy())
+        offs = np32(g.offsets)
+        if isinstance(g, AsphericDiskMirrorGroup):
+            P, N, D, W = O.sample_group(as_key(sub), 0, surf, offs, wl.n_samples,
+                                        radii=np32(g.radii))
+        else:
+            P, N, D, W = O.sample_group(as_key(sub), 0, surf, offs, wl.n_samples,
+                                        vertices=np32(g.vertices))
+        rot = np32(g.rotations).copy()
+        if wl.tilt_sigma_deg > 0:
+            rng = np.random.default_rng(wl.tilt_seed)
+            rot[:, 0:2] += rng.normal(0.0, wl.tilt_sigma_deg, (len(g), 2)).astype(np.float32)
+        parts.append((P, N, D, W, np32(g.positions), rot, np32(g.perturbation_scale)))
+    cat = lambda i: np.concatenate([p[i] for p in parts], 0)  # noqa: E731
+    return O.Scene(points=cat(0), normals=cat(1), delta=cat(2), weights=cat(3), positions=cat(4),
+                   rotations=cat(5), scale=cat(6), sensor=oracle_sensor(wl.sensor),
+                   obstructions=oracle_obstructions(group_obstructions(wl.obstructions)),
+                   mirrors=[])

@@ -44,3 +44,50 @@ def test_cpu_ref_matches_oracle(case):
     tot = np.abs(ref).sum()
     assert tot > 0
     assert np.abs(img - ref).sum() <= 1e-4 * tot, (name, np.abs(img - ref).sum() / tot)
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_cpu_ref_fp64_run_matches_oracle_fp64(case):
+    """The FP64 instantiation against the numpy oracle's fp64 path.  Both run
+    in double, but the library's scene descriptor holds fp32 numbers -- the
+    stage mirrors' rotation matrices among them (sin 180 deg rounds to
+    -8.7e-8 in fp32) -- so rays near a pixel edge may differ: every pixel
+    agrees within its edge-band mass, and the image to 1e-3 in L1."""
+    import copy
+    name, sc, src, vals, st = _scenes()[case]
+    # the descriptors hold surface parameters in fp32: give the oracle the
+    # same numbers
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    sc.mirrors = [copy.copy(m) for m in sc.mirrors]
+    for m in sc.mirrors:
+        m.surface = O.Surface(f32(m.surface.curvature), f32(m.surface.conic),
+                              np.asarray(m.surface.aspheric, np.float32).astype(np.float64))
+    keep = []
+    s = cpu_ref.scene_struct(sc, keep)
+    img, band, _ = cpu_ref.render_band(s, src, vals, st, sc.sensor.n_pixels, 4, 64, 1e-3)
+    ref = O.render(sc, src, vals, st, "fp64", "image")
+    tot = np.abs(ref).sum()
+    assert tot > 0
+    assert np.all(np.abs(img - ref) <= band + 1e-9 * np.abs(ref).max()), name
+    assert np.abs(img - ref).sum() <= 1e-3 * tot, (name, np.abs(img - ref).sum() / tot)
+    if case == 0:  # no stage mirrors: the same numbers in, the same image out
+        assert np.abs(img - ref).sum() <= 1e-9 * tot
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_band_image_bounds_fp32_vs_fp64(case):
+    """Every pixel's fp32-vs-fp64 difference lies within its band mass: the
+    rays that bin differently are all within the band of a boundary."""
+    name, sc, src, vals, st = _scenes()[case]
+    keep = []
+    s = cpu_ref.scene_struct(sc, keep)
+    P = sc.sensor.n_pixels
+    i64, b64, stats = cpu_ref.render_band(s, src, vals, st, P, 4, 64, 1e-3)
+    i32 = cpu_ref.render(s, src, vals, st, P, 4, 32)
+    assert stats["traced"] == len(src) * sc.points.shape[0] * sc.points.shape[1]
+    assert 0 < stats["binned"] <= stats["traced"]
+    slack = 1e-5 * np.abs(i64) + 1e-7 * np.abs(i64).max()
+    assert np.all(np.abs(i32 - i64) <= b64 + slack), name
+    # a band of zero width explains nothing; a planted offset of the fp32
+    # image is caught
+    assert not np.all(np.abs(i32 * (1 + 1e-3) - i64) <= b64 + slack)

@@ -43,6 +43,18 @@ def _trace_and_compare(tel, src, vals, stype, label, rtol=1e-5, launch=None):
     print(f"[{label}] rays={n} band={int(band.sum())} pix_mismatch_in_band="
           f"{int(((pk != po) & band).sum())} pix_mismatch_outside={int(bad_pix.sum())} "
           f"val_mismatch_outside={int(bad_val.sum())}", flush=True)
+    # edge-band evidence: how far the kernel's sensor coordinate is from the
+    # fp64 one (pixel pitch / hex inradius units), and the fp64 distance of
+    # every ray binned differently to its nearest binning boundary
+    unit = (sc.sensor.params()[2] if isinstance(sc.sensor, O.SquareSensorSpec)
+            else sc.sensor.inradius)
+    ok = (np.abs(xy[:, 0]) < 1e9) & (np.abs(x64) < 1e9) & (vo != 0) & ((po >= 0) | (pk >= 0))
+    cerr = (np.hypot(xy[ok, 0] - x64[ok], xy[ok, 1] - y64[ok]) / unit).max() if ok.any() else 0.0
+    marg = O.edge_margin(x64, y64, sc.sensor)
+    mm = (pk != po) & (np.abs(x64) < 1e9)
+    print(f"[{label}] kernel vs fp64 sensor coordinate: max error {cerr:.2e} px; rays binned "
+          f"differently from ref32: {int((pk != po).sum())}, their max fp64 edge distance "
+          f"{marg[mm].max() if mm.any() else 0:.2e} px", flush=True)
     assert bad_pix.sum() == 0, np.nonzero(bad_pix)[0][:10]
     assert bad_val.sum() == 0, np.nonzero(bad_val)[0][:10]
     # image: oracle ray values accumulated at the kernel's pixel ids
