@@ -17,6 +17,7 @@ IACT_MAX_ASPHERIC = 8
 IACT_MAX_POLY_VERTS = 8
 IACT_MAX_SURFACES = 16
 IACT_MIRROR_STRIDE = 40
+IACT_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 IACT_OK = 0
 IACT_ERR_INVALID = -1
@@ -128,6 +129,9 @@ _SIGNATURES = {
     "iact_trace_fwd": (c_int32, [POINTER(IactScene), c_void_p, c_void_p, c_int32, c_int32,
                                  c_int32, POINTER(IactLaunch), c_void_p, c_size_t, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
+    "iact_trace_fwd_noalloc": (c_int32, [POINTER(IactScene), c_void_p, c_void_p, c_int32,
+                                         c_int32, c_int32, POINTER(IactLaunch), c_void_p,
+                                         c_void_p, c_void_p, c_void_p]),
     "iact_bwd_workspace_size": (c_size_t, [POINTER(IactScene), c_int32, POINTER(IactLaunch)]),
     "iact_trace_bwd": (c_int32, [POINTER(IactScene), c_void_p, c_void_p, c_int32, c_int32,
                                  c_void_p, POINTER(IactLaunch), c_void_p, c_size_t,

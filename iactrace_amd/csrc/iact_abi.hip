@@ -750,4 +750,66 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
   return e == hipSuccess ? IACT_OK : hip_error(e, "k_trace(debug)");
 }
 
+// ------------------------------------------------------- no-allocation entry
+// Scratch of iact_trace_fwd_noalloc: a file-scope device array, one copy per
+// GPU (each device's code object holds its own), kept until the process
+// ends.  Host code uses only the per-device address from hipGetSymbolAddress
+// (the array's name is a host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[IACT_NOALLOC_SCRATCH_BYTES];
+
+namespace iact {
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+// Device address of the current device's scratch array, looked up on the
+// first call on that device (with it current) and kept.
+static int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) return set_error(IACT_ERR_UNSUPPORTED, "device ordinal %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(IACT_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return IACT_OK;
+}
+}  // namespace iact
+
+int32_t iact_trace_fwd_noalloc(const iact_scene* scene, const double* sources,
+                               const float* values, int32_t n_sources, int32_t source_type,
+                               int32_t mode, const iact_launch* launch, float* out,
+                               float* debug_xy, int32_t* debug_pixel, void* stream) {
+  int rc = validate_scene(scene);
+  if (rc) return rc;
+  if (n_sources < 0) return set_error(IACT_ERR_INVALID, "negative source count");
+  if (mode != IACT_MODE_IMAGE && mode != IACT_MODE_RESPONSE && mode != IACT_MODE_DEBUG)
+    return set_error(IACT_ERR_INVALID, "unknown mode %d", mode);
+  // the workspace the render needs; an image render whose default layout
+  // needs more than the scratch holds (the source-major record table, 64 B
+  // per sample) takes the sample-major layout, which has none
+  iact_launch L = launch ? *launch : iact_launch{};
+  // (a size of 0 from an invalid launch: iact_trace_fwd reports the error)
+  size_t need = iact_workspace_size(scene, n_sources, mode, &L);
+  if (need > (size_t)IACT_NOALLOC_SCRATCH_BYTES && mode == IACT_MODE_IMAGE && L.layout == 0) {
+    L.layout = 1;
+    need = iact_workspace_size(scene, n_sources, mode, &L);
+  }
+  if (need > (size_t)IACT_NOALLOC_SCRATCH_BYTES)
+    return set_error(IACT_ERR_UNSUPPORTED,
+                     "render needs %zu bytes of scratch; iact_trace_fwd_noalloc holds %zu "
+                     "(use iact_trace_fwd with a caller workspace)",
+                     need, (size_t)IACT_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc) return rc;
+  return iact_trace_fwd(scene, sources, values, n_sources, source_type, mode, &L, ws,
+                        (size_t)IACT_NOALLOC_SCRATCH_BYTES, out, debug_xy, debug_pixel, stream);
+}
+
 }  // extern "C"

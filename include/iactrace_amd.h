@@ -249,6 +249,25 @@ int32_t iact_trace_fwd(const iact_scene* scene, const double* sources, const flo
                        const iact_launch* launch, void* workspace, size_t workspace_bytes,
                        float* out, float* debug_xy, int32_t* debug_pixel, void* stream);
 
+/* ---- forward render without allocation -----------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): iact_trace_fwd with the workspace taken from a
+ * library-owned static device array of IACT_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends) instead of a caller buffer.  Arguments,
+ * outputs and errors as iact_trace_fwd; an image render whose default
+ * layout needs more scratch than that (the source-major layout's record
+ * table, 64 B per facet sample) runs the sample-major layout instead, and a
+ * render that still does not fit fails with IACT_ERR_UNSUPPORTED.  Calls no
+ * allocating routine.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (on different streams or host threads) race and
+ * must be ordered by the caller.                                            */
+#define IACT_NOALLOC_SCRATCH_BYTES (32u << 20)
+int32_t iact_trace_fwd_noalloc(const iact_scene* scene, const double* sources,
+                               const float* values, int32_t n_sources, int32_t source_type,
+                               int32_t mode, const iact_launch* launch, float* out,
+                               float* debug_xy, int32_t* debug_pixel, void* stream);
+
 /* ---- adjoint (reverse-mode AD of render, which the reference gets from
  * jax.grad through render.py:174-220) --------------------------------------
  * Given dL/dimage, writes dL/d(stage-0 rotations, positions), dL/d(sources,
