@@ -192,18 +192,20 @@ def cpu_baseline(wl, tel, target_s=15.0):
                       f"with fp64 directions, gcc -O3 -march=x86-64-v3 -fopenmp"}, img, n
 
 
-def image_parity(ia, wl, tel, src, vals, band=1e-3):
+def image_parity(ia, wl, tel, src, vals):
     """The GPU image of the whole workload against oracle/cpu_ref's FP64 run
     of the same samples (every step in double, image summed in double), on
     the host cores: relative L1 error, the largest relative error of a
     significant pixel (>= 1e-3 of the peak) and the same after taking away
-    the pixel's edge-band mass (|value| of the FP64 rays within `band` px of
-    a binning boundary, at their pixel and the one across the nearest edge:
-    the rays an fp32 coordinate may put on either side)."""
+    the pixel's edge-band mass (|value| of the FP64 rays within the edge band
+    of a binning boundary -- 1e-3 px or 2.5e-5 m, tests.helpers.band_units --
+    at their pixel and the one across the nearest edge: the rays an fp32
+    coordinate may put on either side)."""
     import numpy as np
     from oracle import cpu_ref
-    from tests.helpers import oracle_scene
+    from tests.helpers import band_units, oracle_scene
     sc = oracle_scene(tel)
+    band = band_units(sc.sensor)
     keep = []
     s = cpu_ref.scene_struct(sc, keep)
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)

@@ -85,8 +85,92 @@ def grown_obstructions(ob, eps):
     g.boxes = [(np.minimum(a, b) - eps, np.maximum(a, b) + eps) for a, b in ob.boxes]
     g.spheres = [(c, r + eps) for c, r in ob.spheres]
     g.oriented_boxes = [(c, np.asarray(h) + eps, R) for c, h, R in ob.oriented_boxes]
-    g.triangles = list(ob.triangles)
+    g.triangles = [_grown_triangle(a, b, c, eps) for a, b, c in ob.triangles]
     return g
+
+
+def _grown_triangle(a, b, c, eps):
+    """The triangle scaled about its incentre so each edge moves out by eps."""
+    a, b, c = (np.asarray(v, np.float64) for v in (a, b, c))
+    la, lb, lc = np.linalg.norm(b - c), np.linalg.norm(c - a), np.linalg.norm(a - b)
+    inc = (la * a + lb * b + lc * c) / (la + lb + lc)
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a))
+    rho = 2.0 * area / (la + lb + lc)
+    k = 1.0 + eps / rho
+    return tuple(inc + k * (v - inc) for v in (a, b, c))
+
+
+def _grown_polygon(verts, eps):
+    """A convex polygon scaled about its vertex mean so that every edge moves
+    by at least |eps| (outwards for eps > 0)."""
+    v = np.asarray(verts, np.float64)
+    c = v.mean(0)
+    d = []
+    for i in range(len(v)):
+        p, q = v[i], v[(i + 1) % len(v)]
+        e = q - p
+        d.append(abs(e[0] * (c[1] - p[1]) - e[1] * (c[0] - p[0])) / np.linalg.norm(e))
+    return c + (v - c) * (1.0 + eps / min(d))
+
+
+EPS32 = 2.0 ** -24
+
+
+def _discriminant_ambiguous(o, d, obs, k=32.0):
+    """Rays (origins o, directions d, float64) whose hit-or-miss decision on a
+    cylinder or sphere the fp32 discriminant b^2 - 4ac of the reference's
+    quadratic (intersections.py:44-81, 195-226) cannot resolve: |disc| below
+    k * 2^-24 * (b^2 + 4a(|oc|^2 + r^2)), the rounding bound of its terms, with
+    the grazing point ahead of the origin and, for a cylinder, within the
+    axial extent (+ r).  For a thin mast tens of metres away this is a
+    millimetre-wide band around its silhouette.  Also the cylinder's cap
+    tests |p|^2 <= r^2 on the cap planes, within their rounding bound."""
+    amb = np.zeros(o.shape[0], bool)
+    dot = lambda u, v: np.sum(u * v, -1)  # noqa: E731
+    for p1, p2, r in obs.cylinders:
+        p1 = np.asarray(p1, np.float64)
+        ax = np.asarray(p2, np.float64) - p1
+        h = np.linalg.norm(ax)
+        ax = ax / h
+        oc = o - p1
+        oca, rda = dot(oc, ax), dot(d, ax)
+        ocp, rdp = oc - oca[:, None] * ax, d - rda[:, None] * ax
+        a, b = dot(rdp, rdp), 2.0 * dot(ocp, rdp)
+        noc2 = dot(oc, oc)
+        disc = b * b - 4.0 * a * (dot(ocp, ocp) - r * r)
+        bound = k * EPS32 * (b * b + 4.0 * a * (noc2 + r * r))
+        tg = -b / (2.0 * a + 1e-300)
+        yg = oca + tg * rda
+        amb |= (np.abs(disc) <= bound) & (tg > 0) & (yg > -r) & (yg < h + r)
+        for plane in (0.0, h):
+            tc = (plane - oca) / (rda + 1e-8)
+            pc = ocp + tc[:, None] * rdp
+            q = dot(pc, pc) - r * r
+            qb = k * EPS32 * (np.sqrt(noc2) + np.abs(tc)) ** 2 + k * EPS32 * r * r
+            amb |= (tc > 0) & (np.abs(q) <= qb)
+    for c, r in obs.spheres:
+        oc = o - np.asarray(c, np.float64)
+        a, b = dot(d, d), 2.0 * dot(oc, d)
+        noc2 = dot(oc, oc)
+        disc = b * b - 4.0 * a * (noc2 - r * r)
+        bound = k * EPS32 * (b * b + 4.0 * a * (noc2 + r * r))
+        amb |= (np.abs(disc) <= bound) & (-b / (2.0 * a) > 0)
+    return amb
+
+
+BAND_METRES = 2.5e-5
+
+
+def band_units(sensor, px_band=1e-3, metres=BAND_METRES):
+    """The edge band in the sensor's margin units (pixel pitches for square
+    sensors, inradii for hexagonal ones): px_band, or `metres` on the sensor
+    when that is wider (sensors with millimetre pixels)."""
+    if isinstance(sensor, O.SquareSensorSpec):
+        x0, y0, dx, dy = sensor.params()
+        unit = min(dx, dy)
+    else:
+        unit = sensor.inradius
+    return max(px_band, metres / unit)
 
 
 def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
@@ -95,14 +179,31 @@ def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
     boundary; a shadow/aperture decision that changes when obstructions and
     stage>=1 apertures are grown or shrunk by ob_eps metres (float64 oracle);
     or one that the reference's fp32 arithmetic (the ref32 oracle, no fused
-    multiply-adds) gets differently from float64 -- e.g. the discriminant
-    b^2 - 4ac of a ray grazing a cylinder, where an evaluation with a fused
-    multiply-add (as the kernels, and XLA where it contracts) may side with
-    float64 instead.
+    multiply-adds) gets differently from float64; or a primary-leg shadow
+    ray whose cylinder / sphere discriminant lies within its fp32 rounding
+    bound (_discriminant_ambiguous): there any two fp32 evaluations -- with
+    or without fused multiply-adds, the kernels' or XLA's -- may disagree.
+
+    The edge band is px_band = 1e-3 pixels (hex-norm units for hexagonal
+    sensors) or 2.5e-5 m on the sensor, whichever is wider (band_units).  The
+    largest distance between a kernel's fp32 sensor coordinate and the
+    float64 one over every per-ray parity case is 8.6e-6 m (the C3 dish,
+    focal length 36 m: 4.1e-4 of its 2.1 cm inradius), so the band is 2.9
+    times that; the rays the kernels actually bin differently lie within
+    9.1e-4 px of an edge on CT3's 1 mm-pixel lid camera and within 1.2e-4 px
+    on the others (each parity case prints both figures).
     """
     x, y, v, _ = O.render(sc, sources, values, source_type, "fp64", "debug")
-    band = O.edge_margin(x, y, sc.sensor) < px_band
+    band = O.edge_margin(x, y, sc.sensor) < band_units(sc.sensor, px_band)
     alive = v != 0
+    if sc.obstructions.cylinders or sc.obstructions.spheres:
+        wp = O.transform_to_world(sc, "fp64")[0]                # (nf, M, 3)
+        nf, M = wp.shape[:2]
+        ns = len(sources)
+        o = np.broadcast_to(wp[:, None], (nf, ns, M, 3)).reshape(-1, 3)
+        d = np.stack([O.source_dirs(wp[f], sources, source_type, "fp64").astype(np.float64)
+                      for f in range(nf)])                    # (nf, ns, M, 3)
+        band |= _discriminant_ambiguous(o, -d.reshape(-1, 3), sc.obstructions)
     _, _, v32, _ = O.render(sc, sources, values, source_type, "ref32", "debug")
     band |= (v32 != 0) != alive
     import copy
@@ -113,6 +214,8 @@ def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
         for m in sc2.mirrors:
             if m.radius is not None:
                 m.radius = m.radius - eps
+            else:
+                m.vertices = _grown_polygon(m.vertices, -eps)
         _, _, v2, _ = O.render(sc2, sources, values, source_type, "fp64", "debug")
         band |= (v2 != 0) != alive
     return band, x, y, v

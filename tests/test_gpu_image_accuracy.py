@@ -14,10 +14,11 @@ same samples (every step in double, image summed in double):
     edge band, which the per-ray tests count;
   * on every significant pixel (>= 1e-3 of the peak), the error that the edge
     band does not explain <= 1e-5 of the pixel: |gpu - fp64| minus the band
-    mass of that pixel (|value| of the FP64 rays within 1e-3 px -- hex-norm
-    units -- of a binning boundary, at their pixel and the one across the
-    nearest edge; the kernels' sensor coordinates stay within 4.1e-4 of the
-    FP64 ones, see the per-ray tests' printed coordinate errors);
+    mass of that pixel (|value| of the FP64 rays within the edge band of a
+    binning boundary -- 1e-3 px or 2.5e-5 m, tests.helpers.band_units -- at
+    their pixel and the one across the nearest edge; the kernels' sensor
+    coordinates stay within 8.6e-6 m of the FP64 ones, see the per-ray tests'
+    printed coordinate errors);
   * the raw per-pixel error and the band counts are printed.
 """
 import numpy as np
@@ -27,11 +28,10 @@ import torch
 import iactrace_amd as ia
 from iactrace_amd import configs
 from oracle import cpu_ref
-from tests.helpers import oracle_scene
+from tests.helpers import band_units, oracle_scene
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
-BAND = 1e-3
 
 
 def image_vs_fp64(img, i64, b64):
@@ -58,7 +58,7 @@ def test_image_vs_fp64_run(name):
     keep = []
     s = cpu_ref.scene_struct(sc, keep)
     i64, b64, st = cpu_ref.render_band(s, wl.sources, wl.values, wl.source_type,
-                                       sc.sensor.n_pixels, 0, 64, BAND)
+                                       sc.sensor.n_pixels, 0, 64, band_units(sc.sensor))
     i32 = cpu_ref.render(s, wl.sources, wl.values, wl.source_type, sc.sensor.n_pixels, 0, 32)
     l1, raw, res, nsig = image_vs_fp64(img.cpu().numpy(), i64, b64)
     l1c, rawc, resc, _ = image_vs_fp64(i32, i64, b64)
