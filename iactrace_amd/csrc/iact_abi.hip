@@ -523,6 +523,33 @@ int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hi
   return IACT_OK;
 }
 
+// The source-major adjoint (k_adj_sm) traces from the image layout's sample
+// records: its workspace is the source-major image trace's (records, then
+// the culling tables with the source sort and the block bounds) followed by
+// the per-facet torques.
+bool bwd_source_major(const iact_scene& sc, int ns, const iact_launch* L, bool want_asph) {
+  if (sc.n_stages > 0 || sc.sensor.splat || want_asph) return false;
+  if (L && (L->layout == 1 || L->deterministic)) return false;
+  if ((long long)sc.facets.n_facets * sc.facets.n_samples >= (1ll << 31)) return false;
+  return ns >= 64 || (L && L->layout == 2);
+}
+
+int adj_sm_layout(const iact_scene& sc, const double* sources, const float* values, int ns,
+                  const iact_launch* L, const DevInfo& dev, TraceArgs& A, LaunchShape& shp,
+                  AdjSmLayout& out) {
+  fill_trace_args(A, sc, sources, values, ns);
+  iact_launch l = L ? *L : iact_launch{};
+  l.layout = 2;
+  int rc = shape_trace(sc, ns, IACT_MODE_IMAGE, &l, A, shp, dev, -1);
+  if (rc) return rc;
+  out.recs_end = base_workspace_bytes(sc, IACT_MODE_IMAGE, shp, A);
+  const CullSpec c{ns, 1, nullptr, true, refine_groups(A, IACT_MODE_IMAGE)};
+  const size_t end = cull_layout(A.nf, A.cull_words, A.cull_all, out.recs_end, c).end;
+  out.torque = align256(end);
+  out.end = out.torque + (size_t)3 * sc.facets.n_facets * sizeof(double);
+  return IACT_OK;
+}
+
 long long refine_groups(const TraceArgs& A, int mode) {
   if (mode != IACT_MODE_IMAGE || A.layout != 2 || !has_obstructions(A.ob)) return 0;
   return ((long long)A.nf * A.M + 63) / 64;

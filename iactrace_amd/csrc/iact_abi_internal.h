@@ -49,6 +49,16 @@ long long refine_groups(const TraceArgs& A, int mode);
 int cull_words_for(const iact_scene& sc, const iact_launch* L, int& words, int& all);
 // Offset of the fp64 image sums after a workspace of `end` bytes.
 size_t image64_offset(size_t end);
+// Source-major adjoint: whether a backward takes it, and its workspace
+// layout (record table, culling tables from recs_end, per-facet torques at
+// `torque`, `end` bytes in all) with the trace arguments and launch shape.
+struct AdjSmLayout {
+  size_t recs_end, torque, end;
+};
+bool bwd_source_major(const iact_scene& sc, int ns, const iact_launch* L, bool want_asph);
+int adj_sm_layout(const iact_scene& sc, const double* sources, const float* values, int ns,
+                  const iact_launch* L, const DevInfo& dev, TraceArgs& A, LaunchShape& shp,
+                  AdjSmLayout& out);
 CullLayout cull_layout(int nf, int words, int all, size_t base, const CullSpec& c);
 int prepare_cull(TraceArgs& A, void* workspace, size_t base, int source_type, hipStream_t st,
                  const CullSpec& c, const int32_t** perm);

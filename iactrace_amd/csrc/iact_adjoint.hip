@@ -257,7 +257,9 @@ IACT_DEV void stage_adjoint(const AdjArgs& B, const StageRec& rec, f3& go, f3& g
 // gradient of the instance the block is working on and is reloaded when the
 // block moves on to the next instance; only rotation gradients, per
 // instance, are produced.
-template <int SRC, int KIND, int SPLAT, bool BATCH, bool STAGES>
+// AXIS: the sensor frame is the world frame (the binning then reads the
+// direction's components instead of three dot products), as in the forward.
+template <int SRC, int KIND, int SPLAT, bool BATCH, bool STAGES, bool AXIS>
 __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   const TraceArgs& A = B.T;
   extern __shared__ float smem[];
@@ -318,6 +320,12 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
     }
     f3 Ap = mk3(0.f, 0.f, 0.f), An = mk3(0.f, 0.f, 0.f);
     float Anrw = 0.f;
+    // hard binning through the primary leg only: the point-source term of
+    // the sample's adjoint is sum_s t_s (n - c_s d_s), t_s = dc_s / |u_s|,
+    // with n fixed in the source loop -- summed as sum t_s and sum t_s c_s d_s
+    constexpr bool SPLIT_AP = SRC == IACT_SOURCE_POINT && !SPLAT && !STAGES;
+    float Tn = 0.f;
+    f3 Wd = mk3(0.f, 0.f, 0.f);
     const int s0 = w.s0, s1 = w.s1;
     const double* src = A.sources + 3 * w.sbase;
     const cptr<float> vals = as_const(A.values + w.sbase);
@@ -368,9 +376,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
           // normalised), so that the pixel is the forward's
           const f3 u = smp.q + e;
           const f3 ru = fma3(-2.f * dot(u, smp.n), smp.n, u);
-          ok = plane_grid<KIND, false>(A.S, Kg, ru, gu_, gv_, inv) && active;
+          ok = plane_grid<KIND, AXIS>(A.S, Kg, ru, gu_, gv_, inv) && active;
         } else {
-          ok = plane_grid<KIND, false>(A.S, Kg, rf, gu_, gv_) && active;
+          ok = plane_grid<KIND, AXIS>(A.S, Kg, rf, gu_, gv_) && active;
         }
         const int pix = grid_pixel<KIND>(A.S, tab, gu_, gv_, ok);
         gv = pix >= 0 ? G[pix] : 0.f;
@@ -392,30 +400,45 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       }
       // now (go, gr, gv) are the adjoints of the primary leg's origin (the
       // sample), reflected direction and value = value_s * c * nrw * shadow
-      Ap = Ap + go;
-      const float dc = gv * value * smp.nrw * shadow;
-      Anrw = fmaf(gv * value * shadow, c, Anrw);
-      f3 gd = dc * smp.n;
+      const float gvs = gv * value * shadow;
+      const float dc = gvs * smp.nrw;
+      if (want_asph) Anrw = fmaf(gvs, c, Anrw);
       An = fma3(dc, d, An);
-      if (SPLAT || STAGES) {
-        // r = d - 2 (d.n) n
-        const float ng = dot(smp.n, gr);
-        gd = gd + fma3(-2.f * ng, smp.n, gr);
-        An = An + fma3(-2.f * ng, d, (-2.f * c) * gr);
-      }
-      float gsx, gsy, gsz;
-      if (SRC == IACT_SOURCE_POINT) {
-        // d = u / |u|, u = p - src
-        const float dd = dot(d, gd);
-        const f3 gu = inv * fma3(-dd, d, gd);
-        Ap = Ap + gu;
-        gsx = -gu.x;
-        gsy = -gu.y;
-        gsz = -gu.z;
+      float gsx = 0.f, gsy = 0.f, gsz = 0.f;
+      if (SPLIT_AP) {
+        // d = u / |u|, u = p - src, dL/dd = dc n: dL/dp = t (n - c d),
+        // t = dc / |u|, and dL/dsrc = -dL/dp
+        const float t = dc * inv;
+        Tn += t;
+        Wd = fma3(t * c, d, Wd);
+        if (want_src) {
+          const f3 gu = t * fma3(-c, d, smp.n);
+          gsx = -gu.x;
+          gsy = -gu.y;
+          gsz = -gu.z;
+        }
       } else {
-        gsx = gd.x;
-        gsy = gd.y;
-        gsz = gd.z;
+        Ap = Ap + go;
+        f3 gd = dc * smp.n;
+        if (SPLAT || STAGES) {
+          // r = d - 2 (d.n) n
+          const float ng = dot(smp.n, gr);
+          gd = gd + fma3(-2.f * ng, smp.n, gr);
+          An = An + fma3(-2.f * ng, d, (-2.f * c) * gr);
+        }
+        if (SRC == IACT_SOURCE_POINT) {
+          // d = u / |u|, u = p - src
+          const float dd = dot(d, gd);
+          const f3 gu = inv * fma3(-dd, d, gd);
+          Ap = Ap + gu;
+          gsx = -gu.x;
+          gsy = -gu.y;
+          gsz = -gu.z;
+        } else {
+          gsx = gd.x;
+          gsy = gd.y;
+          gsz = gd.z;
+        }
       }
       if (want_src || want_val) {
         const float a = want_src ? wave_sum(active ? gsx : 0.f) : 0.f;
@@ -434,6 +457,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
         }
       }
     }
+    if (SPLIT_AP) Ap = fma3(Tn, smp.n, mk3(-Wd.x, -Wd.y, -Wd.z));
     if (!active) {
       Ap = mk3(0.f, 0.f, 0.f);
       An = mk3(0.f, 0.f, 0.f);
@@ -530,11 +554,18 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
 
 using AdjKernel = void (*)(AdjArgs);
 
+template <int SRC, int KIND, int SPLAT, bool AXIS>
+static AdjKernel pick_adj_ax(const AdjArgs& B) {
+  if (B.T.item_prefix) return k_trace_adj<SRC, KIND, SPLAT, true, false, AXIS>;
+  return B.T.n_stages > 0 ? k_trace_adj<SRC, KIND, SPLAT, false, true, AXIS>
+                          : k_trace_adj<SRC, KIND, SPLAT, false, false, AXIS>;
+}
+
 template <int SRC, int KIND, int SPLAT>
 static AdjKernel pick_adj_t(const AdjArgs& B) {
-  if (B.T.item_prefix) return k_trace_adj<SRC, KIND, SPLAT, true, false>;
-  return B.T.n_stages > 0 ? k_trace_adj<SRC, KIND, SPLAT, false, true>
-                          : k_trace_adj<SRC, KIND, SPLAT, false, false>;
+  // splat sensors bin through plane_xy, which has no axis variant
+  return (B.T.S.axis && !SPLAT) ? pick_adj_ax<SRC, KIND, SPLAT, true>(B)
+                                : pick_adj_ax<SRC, KIND, SPLAT, false>(B);
 }
 
 template <int SRC>
@@ -576,7 +607,16 @@ size_t iact_bwd_workspace_size(const iact_scene* scene, int32_t n_sources,
   LaunchShape shp;
   if (shape_trace(*scene, n_sources, IACT_MODE_DEBUG, launch, A, shp, dev, -1)) return 0;
   const CullSpec c{n_sources, 1, nullptr, true};
-  return cull_layout(A.nf, A.cull_words, A.cull_all, 0, c).end;
+  size_t need = cull_layout(A.nf, A.cull_words, A.cull_all, 0, c).end;
+  // (the source-major adjoint is not taken when aspheric gradients are
+  // requested; the size covers both)
+  if (bwd_source_major(*scene, n_sources, launch, false)) {
+    TraceArgs B;
+    AdjSmLayout lay;
+    if (adj_sm_layout(*scene, nullptr, nullptr, n_sources, launch, dev, B, shp, lay)) return 0;
+    if (lay.end > need) need = lay.end;
+  }
+  return need;
 }
 
 int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const float* values,
@@ -620,6 +660,48 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
   if (rc) return rc;
   AdjArgs B;
   memset(&B, 0, sizeof(B));
+  if (bwd_source_major(sc, n_sources, launch, grads->aspheric != nullptr)) {
+    // lanes = sources over the image trace's sample records (k_adj_sm)
+    LaunchShape shp;
+    AdjSmLayout lay;
+    TraceArgs& A = B.T;
+    rc = adj_sm_layout(sc, sources, values, n_sources, launch, dev, A, shp, lay);
+    if (rc) return rc;
+    if (lay.end > 0 && (!workspace || workspace_bytes < lay.end))
+      return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", lay.end,
+                       workspace_bytes);
+    const size_t tab = sc.sensor.kind == IACT_SENSOR_HEX
+                           ? (size_t)sc.sensor.table_q * sc.sensor.table_r * sizeof(int32_t)
+                           : 0;
+    shp.lds = (size_t)sc.sensor.n_pixels * sizeof(float) + tab;
+    if (shp.lds > 65536)
+      return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS",
+                       shp.lds);
+    const CullSpec c{n_sources, 1, nullptr, true, refine_groups(A, IACT_MODE_IMAGE)};
+    rc = prepare_cull(A, workspace, lay.recs_end, source_type, st, c, &B.src_perm);
+    if (rc) return rc;
+    A.recs = (const float*)workspace;
+    e = launch_records(A, (float*)workspace, st);
+    if (e != hipSuccess) return hip_error(e, "k_sample_records");
+    if (A.gbounds) {
+      e = launch_group_bounds(A, (float*)A.gbounds, st);
+      if (e != hipSuccess) return hip_error(e, "k_group_bounds");
+    }
+    B.torque = (double*)((char*)workspace + lay.torque);
+    e = launch_fill_d(B.torque, 3ll * nf, 0.0, st);
+    if (e != hipSuccess) return hip_error(e, "clear torques");
+    B.grad_image = grad_image;
+    B.g_pos = grads->positions;
+    B.g_src = grads->sources;
+    B.g_val = grads->values;
+    e = launch_adjoint_sm(B, shp, source_type, st);
+    if (e != hipSuccess) return hip_error(e, "k_adj_sm");
+    if (grads->rotations) {
+      e = launch_torque_to_euler(B.torque, sc.facets.rotations, nf, grads->rotations, st);
+      if (e != hipSuccess) return hip_error(e, "k_torque_to_euler");
+    }
+    return IACT_OK;
+  }
   fill_trace_args(B.T, sc, sources, values, n_sources);
   LaunchShape shp;
   rc = shape_trace(sc, n_sources, IACT_MODE_DEBUG, launch, B.T, shp, dev, -1);

@@ -207,6 +207,18 @@ static int batch_need(const iact_scene* scene, const iact_batch* batch, const ia
   rc = cull_words_for(*scene, launch, words, all);
   if (rc) return rc;
   need = batch_source_major(scene, launch) ? instance_need(scene, max_ns, words, all) : 0;
+  if (bwd_source_major(*scene, max_ns, launch, false)) {
+    // the backward instance by instance through iact_trace_bwd
+    DevInfo dev;
+    rc = device_info(dev);
+    if (rc) return rc;
+    TraceArgs A;
+    LaunchShape shp;
+    AdjSmLayout lay;
+    rc = adj_sm_layout(*scene, nullptr, nullptr, max_ns, launch, dev, A, shp, lay);
+    if (rc) return rc;
+    if (lay.end > need) need = lay.end;
+  }
   const size_t bat = image64_offset(batch_tables_end(scene, batch, words, all, total)) +
                      (size_t)batch->n_instances * scene->sensor.n_pixels * sizeof(double);
   if (bat > need) need = bat;
@@ -357,6 +369,35 @@ int32_t iact_trace_batched_bwd(const iact_scene* scene, const iact_batch* batch,
                                int32_t source_type, const float* grad_images,
                                const iact_launch* launch, void* workspace,
                                size_t workspace_bytes, double* grad_rotations, void* stream) {
+  if (scene && batch && batch_source_major(scene, launch) && scene->n_stages == 0) {
+    // instance by instance with the plain backward (the source-major
+    // adjoint wherever it applies), sharing the workspace
+    if (source_type != IACT_SOURCE_POINT && source_type != IACT_SOURCE_PARALLEL)
+      return set_error(IACT_ERR_INVALID, "unknown source type %d", source_type);
+    size_t need = 0;
+    int rc = batch_need(scene, batch, launch, need);  // validates scene and batch
+    if (rc) return rc;
+    if (!workspace || workspace_bytes < need)
+      return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                       workspace_bytes);
+    if (!grad_images || !grad_rotations)
+      return set_error(IACT_ERR_INVALID, "null image or rotation gradients");
+    const long long P = scene->sensor.n_pixels;
+    const int nf = scene->facets.n_facets;
+    const int64_t* h = batch->host_source_offsets;
+    for (int i = 0; i < batch->n_instances; ++i) {
+      iact_scene si = *scene;
+      si.facets.rotations = batch->rotations + (size_t)i * 3 * nf;
+      iact_grads gi;
+      memset(&gi, 0, sizeof(gi));
+      gi.rotations = grad_rotations + (size_t)i * 3 * nf;
+      rc = iact_trace_bwd(&si, batch->sources + 3 * h[i], batch->values + h[i],
+                          (int32_t)(h[i + 1] - h[i]), source_type, grad_images + (size_t)i * P,
+                          launch, workspace, workspace_bytes, &gi, stream);
+      if (rc) return rc;
+    }
+    return IACT_OK;
+  }
   TraceArgs A;
   LaunchShape shp;
   bool empty;

@@ -71,9 +71,20 @@ struct AdjArgs {
   const int32_t* surface_index;
   const float* offsets;
   const int32_t* src_perm;  // sorted -> caller source index (NULL: not sorted)
+  double* torque;           // source-major adjoint: (nf, 3) per-facet sums of
+                            // q x dL/dp + n x dL/dn (workspace)
 };
 hipError_t launch_adjoint(const AdjArgs& B, const LaunchShape& L, int source_type,
                           hipStream_t st);
+// Source-major adjoint of a primary-only hard-binned image render (sample
+// records in B.T.recs, laid out as the source-major forward's): per-facet
+// torques into B.torque, positions / source / value gradients when set.
+hipError_t launch_adjoint_sm(const AdjArgs& B, const LaunchShape& L, int source_type,
+                             hipStream_t st);
+// g_rot[f] = (pi / 180) (w_a, w_b, w_c) . torque[f]: the rotation gradient of
+// each facet (Euler degrees) from its torque, w_k the rotation axis of angle k.
+hipError_t launch_torque_to_euler(const double* torque, const float* rotations, int nf,
+                                  double* g_rot, hipStream_t st);
 const void* pick_adjoint_kernel(const AdjArgs& B, int source_type);
 
 }  // namespace iact

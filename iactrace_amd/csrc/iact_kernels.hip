@@ -925,6 +925,248 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   }
 }
 
+// Adjoint of the source-major image trace for primary-only, hard-binned
+// renders (the reference gets it from jax.grad through render.py:174-220).
+// Lanes, items, records, obstruction runs (PART 1 / PART 2) and the per-ray
+// arithmetic up to the pixel are k_trace_sm's, so every ray lands on the
+// pixel the forward gave it; the pixel's image gradient G is read from LDS.
+// With hard binning only the ray value v = value (-d.n) / w * shadow
+// (render.py:141) carries a derivative: dc = dL/d(d.n) = G value shadow nrw,
+// dL/dn = dc d and, for point sources (d = u / |u|, u = p - src),
+// dL/dp = dc / |u| (n - (d.n) d).  Per sample each lane sums over its SPL
+// sources
+//   An = sum dc d,  Tn = sum dc / |u|,  Wd = sum dc / |u| (d.n) d,
+// so that the lane's share of dL/dp is Tn n - Wd.  A rotation moves p - pos
+// and n by w x (.) with w the angle's rotation axis (mirrors.py:64-79:
+// n = R n_l / |n_l|, so the normalisation drops out), hence
+// dL/dangle = w . sum_m (q x dL/dp + n x dL/dn) over the facet's samples.
+// That torque is linear in the per-lane sums, so each lane keeps its own share
+// over a facet run and the wave reduces it (and dL/dpos) once per run; the
+// fp64 per-facet sums go to k_torque_to_euler, which dots them with the axes.
+// Source and value gradients are summed per lane (per source) over an item's
+// samples.
+template <int SRC, int KIND, bool AXIS, int OBST, int SPL, int PART>
+__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
+  const TraceArgs& A = B.T;
+  extern __shared__ float smem[];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
+  const int lane = lane_id();
+  const int P = A.S.n_pixels;
+  float* G = smem;
+  int32_t* tab_lds = (int32_t*)(smem + P);
+  for (int i = threadIdx.x; i < P; i += blockDim.x) G[i] = B.grad_image[i];
+  if (KIND == IACT_SENSOR_HEX)
+    for (int i = threadIdx.x; i < A.S.tq * A.S.tr; i += blockDim.x) tab_lds[i] = A.S.table[i];
+  __syncthreads();
+  const int32_t* table = tab_lds;
+  const long long total = (long long)A.nf * A.M;
+  const bool lean = as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u &&
+                    !A.S.edge && (KIND != IACT_SENSOR_SQUARE || A.S.v_scale == 1.f);
+  const bool want_src = B.g_src != nullptr, want_val = B.g_val != nullptr;
+  const bool want_pos = B.g_pos != nullptr;
+  for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
+       item += (long long)grid_blocks() * wpb) {
+    const uint32_t g = fdiv((uint32_t)item, A.d_sc);
+    const int c = (int)((uint32_t)item - g * A.n_sc);
+    double sx[SPL], sy[SPL], sz[SPL];
+    float value[SPL], gval[SPL];
+    f3 gsrc[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      const int s = (c * SPL + k) * 64 + lane;
+      const int sl = s < A.ns ? s : A.ns - 1;
+      sx[k] = A.sources[3 * sl];
+      sy[k] = A.sources[3 * sl + 1];
+      sz[k] = A.sources[3 * sl + 2];
+      value[k] = s < A.ns ? A.values[sl] : 0.f;
+      gval[k] = 0.f;
+      gsrc[k] = mk3(0.f, 0.f, 0.f);
+    }
+    const int j0 = (int)g * A.sample_group;
+    const int j1 = (int)min((long long)j0 + A.sample_group, total);
+    const float4* recs = (const float4*)A.recs;
+    FacetCull fc[SPL];
+    f3 e[SPL];
+    bool refined = false;
+    int rlist[SPL];
+    uint64_t rbits[SPL];
+    // one sample's rays (SPL per lane): the forward's trace up to the pixel,
+    // then the lane's share of the sample's torque and dL/dp (tq, gp)
+    auto adjoint_one = [&](auto sh, auto gen, const RecRaw& rec, f3& tq, f3& gp) {
+      constexpr int OB = decltype(sh)::value ? OBST : 0;
+      constexpr bool GEN = decltype(gen)::value;
+      constexpr bool RM = OB == 0;
+      const Sample smp0 = unpack_record(rec);
+      float san_x = 0.f, san_y = 0.f, san_z = 0.f, st = 0.f;
+      float sw_x = 0.f, sw_y = 0.f, sw_z = 0.f;
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) {
+        Sample smp = smp0;
+        set_facet(smp, fc[k]);
+        if (OB) {
+          smp.rmode = refined;
+          smp.rlist = rlist[k];
+          smp.rbits = rbits[k];
+        }
+        // the forward's arithmetic (trace_ray): the same pixel
+        float inv, c2u, rs;
+        f3 u, d, r;
+        if (SRC == IACT_SOURCE_POINT) {
+          u = smp.q + e[k];
+          inv = frsq(dot(u, u));
+          c2u = RM ? dot(u, smp.m) : -2.f * dot(u, smp.n);
+          r = fma3(c2u, smp.n, u);
+          rs = inv;
+          d = inv * u;
+        } else {
+          d = primary_direction<SRC>(smp, e[k], inv);
+          c2u = RM ? dot(d, smp.m) : -2.f * dot(d, smp.n);
+          r = fma3(c2u, smp.n, d);
+          rs = 1.f;
+          u = d;
+        }
+        const float shadow = primary_shadow<OB>(A, smp, d, true);
+        float gu, gv;
+        const bool ok = plane_grid<KIND, AXIS, GEN>(A.S, smp.G, r, gu, gv, rs);
+        const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
+        const float gp_pix = pix >= 0 ? G[pix] : 0.f;
+        // a = G value shadow; dc = a nrw is applied per sample
+        const float a = OB ? gp_pix * value[k] * shadow : gp_pix * value[k];
+        if (want_val) {
+          // dL/dvalue = G v / value = G shadow (d.m) nrwh
+          const float gs = OB ? gp_pix * shadow : gp_pix;
+          gval[k] = fmaf(gs * (c2u * inv), smp.nrwh, gval[k]);
+        }
+        if (SRC == IACT_SOURCE_POINT) {
+          const float k1 = a * inv;  // dc / |u| before nrw
+          san_x = fmaf(k1, u.x, san_x);
+          san_y = fmaf(k1, u.y, san_y);
+          san_z = fmaf(k1, u.z, san_z);
+          st += k1;
+          // (d.n) d = -c2u / 2 inv^2 u: the factor -1/2 is applied per sample
+          const float k2 = k1 * c2u * (inv * inv);
+          sw_x = fmaf(k2, u.x, sw_x);
+          sw_y = fmaf(k2, u.y, sw_y);
+          sw_z = fmaf(k2, u.z, sw_z);
+          if (want_src) {
+            // dL/dsrc = -dL/du = -(dc / |u|) (n - (d.n) d)
+            const float tsrc = k1 * smp.nrw;
+            const float cdn = -0.5f * c2u * inv;
+            const f3 gu3 = tsrc * fma3(-cdn, d, smp.n);
+            gsrc[k] = gsrc[k] - gu3;
+          }
+        } else {
+          san_x = fmaf(a, d.x, san_x);
+          san_y = fmaf(a, d.y, san_y);
+          san_z = fmaf(a, d.z, san_z);
+          if (want_src) {
+            // d = the source vector as given: dL/dsrc = dc n
+            gsrc[k] = fma3(a * smp.nrw, smp.n, gsrc[k]);
+          }
+        }
+      }
+      // the lane's share of the sample's torque: linear in the lane's sums,
+      // so the wave reduces only once per facet run
+      const float nrw = smp0.nrw;
+      const f3 An = nrw * mk3(san_x, san_y, san_z);
+      if (SRC == IACT_SOURCE_POINT) {
+        const float Tn = nrw * st;
+        const f3 Wd = (-0.5f * nrw) * mk3(sw_x, sw_y, sw_z);
+        gp = fma3(Tn, smp0.n, mk3(-Wd.x, -Wd.y, -Wd.z));
+        tq = cross(smp0.q, gp) + cross(smp0.n, An);
+      } else {
+        gp = mk3(0.f, 0.f, 0.f);
+        tq = cross(smp0.n, An);
+      }
+    };
+    for (int j = j0; j < j1;) {
+      const int f = (int)fdiv((uint32_t)j, A.d_m);
+      const int jend = min((f + 1) * A.M, j1);
+      int any = 0;
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) {
+        fc[k] = OBST ? facet_cull(A, f, min(c * SPL + k, (A.ns - 1) >> 6)) : FacetCull{f, 0u, 0, 0};
+        any |= fc[k].any;
+      }
+      const bool shadow = OBST && any;
+      if (OBST && (PART == 1 ? shadow : !shadow)) {
+        j = jend;
+        continue;
+      }
+      refined = false;
+      if (OBST && PART == 2 && A.gbounds && (j >> 6) == ((jend - 1) >> 6))
+        refined = refine_candidates<SPL>(A, f, c, j >> 6, lane, fc, rlist, rbits);
+      const cptr<float> fp = as_const(A.positions) + 3 * f;
+      const f3 pos = mk3(fp[0], fp[1], fp[2]);
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+      const float4* rp = recs + 4 * (size_t)j;
+      RecRaw cur = fetch_record(rp);
+      f3 T = mk3(0.f, 0.f, 0.f), Gp = mk3(0.f, 0.f, 0.f);
+      auto run = [&](auto sh, auto gen) {
+        for (; j < jend; ++j) {
+          rp += 4;
+          const RecRaw nxt = fetch_record(rp);
+          f3 tq, gp;
+          adjoint_one(sh, gen, cur, tq, gp);
+          T = T + tq;
+          Gp = Gp + gp;
+          cur = nxt;
+        }
+      };
+      if (OBST && PART == 2) {
+        run(std::true_type{}, std::true_type{});
+      } else {
+        if (lean)
+          run(std::false_type{}, std::false_type{});
+        else
+          run(std::false_type{}, std::true_type{});
+      }
+      T = mk3(wave_sum(T.x), wave_sum(T.y), wave_sum(T.z));
+      if (want_pos) Gp = mk3(wave_sum(Gp.x), wave_sum(Gp.y), wave_sum(Gp.z));
+      if (lane == 0) {
+        unsafeAtomicAdd(B.torque + 3 * f, (double)T.x);
+        unsafeAtomicAdd(B.torque + 3 * f + 1, (double)T.y);
+        unsafeAtomicAdd(B.torque + 3 * f + 2, (double)T.z);
+        if (want_pos) {
+          unsafeAtomicAdd(B.g_pos + 3 * f, (double)Gp.x);
+          unsafeAtomicAdd(B.g_pos + 3 * f + 1, (double)Gp.y);
+          unsafeAtomicAdd(B.g_pos + 3 * f + 2, (double)Gp.z);
+        }
+      }
+    }
+    // the lanes' sources: gradients to the caller's source order
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      const int s = (c * SPL + k) * 64 + lane;
+      if (s < A.ns && (want_val || want_src)) {
+        const long long so = B.src_perm ? (long long)as_const(B.src_perm)[s] : (long long)s;
+        if (want_val && gval[k] != 0.f) unsafeAtomicAdd(B.g_val + so, (double)gval[k]);
+        if (want_src) {
+          unsafeAtomicAdd(B.g_src + 3 * so, (double)gsrc[k].x);
+          unsafeAtomicAdd(B.g_src + 3 * so + 1, (double)gsrc[k].y);
+          unsafeAtomicAdd(B.g_src + 3 * so + 2, (double)gsrc[k].z);
+        }
+      }
+    }
+  }
+}
+
+// dL/d(Euler degrees) of each facet from its torque: R = Rz(c) Ry(b) Rx(a),
+// dR/da R^T = [Rz Ry e_x]x, dR/db R^T = [Rz e_y]x, dR/dc R^T = [e_z]x.
+__global__ void k_torque_to_euler(const double* torque, const float* rotations, int nf,
+                                  double* g_rot) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double k = 3.14159265358979323846 / 180.0;
+  const double b = (double)rotations[3 * f + 1] * k, cz = (double)rotations[3 * f + 2] * k;
+  const double sb = sin(b), cb = cos(b), sc = sin(cz), cc = cos(cz);
+  const double tx = torque[3 * f], ty = torque[3 * f + 1], tz = torque[3 * f + 2];
+  g_rot[3 * f] = k * (cc * cb * tx + sc * cb * ty - sb * tz);
+  g_rot[3 * f + 1] = k * (-sc * tx + cc * ty);
+  g_rot[3 * f + 2] = k * tz;
+}
+
 // Response matrix: each block owns one source at a time, accumulates its
 // image row in LDS over every (facet, sample), then writes the full row
 // (render.py:271-324).  World samples come from the precomputed SoA.
@@ -1122,6 +1364,56 @@ hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_
     if (e != hipSuccess) return e;
   }
   return launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR), A, L, st);
+}
+
+template <int SRC, int KIND, bool AXIS, int SPL>
+static const void* pick_adj_sm_spl(const AdjArgs& B, int which) {
+  if (has_obstructions(B.T.ob)) {
+    if (which == TRACE_SOURCE_MAJOR_FREE) return (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL, 1>;
+    return (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL, 2>;
+  }
+  return (const void*)k_adj_sm<SRC, KIND, AXIS, 0, SPL, 0>;
+}
+template <int SRC, int KIND>
+static const void* pick_adj_sm_k(const AdjArgs& B, int which) {
+  const bool ax = B.T.S.axis != 0;
+  if (B.T.sm_spl == 2)
+    return ax ? pick_adj_sm_spl<SRC, KIND, true, 2>(B, which)
+              : pick_adj_sm_spl<SRC, KIND, false, 2>(B, which);
+  return ax ? pick_adj_sm_spl<SRC, KIND, true, 1>(B, which)
+            : pick_adj_sm_spl<SRC, KIND, false, 1>(B, which);
+}
+template <int SRC>
+static const void* pick_adj_sm(const AdjArgs& B, int which) {
+  return B.T.S.kind == IACT_SENSOR_SQUARE ? pick_adj_sm_k<SRC, IACT_SENSOR_SQUARE>(B, which)
+                                          : pick_adj_sm_k<SRC, IACT_SENSOR_HEX>(B, which);
+}
+
+hipError_t launch_adjoint_sm(const AdjArgs& B, const LaunchShape& L, int source_type,
+                             hipStream_t st) {
+  if (has_obstructions(B.T.ob) && !B.T.cull) return hipErrorInvalidValue;
+  if (B.T.S.splat || B.T.n_stages > 0 || B.T.layout != 2 || !B.T.recs || !B.torque)
+    return hipErrorInvalidValue;
+  auto go = [&](int which) {
+    AdjArgs b = B;
+    void* args[] = {&b};
+    const void* k = source_type == IACT_SOURCE_POINT ? pick_adj_sm<IACT_SOURCE_POINT>(B, which)
+                                                     : pick_adj_sm<IACT_SOURCE_PARALLEL>(B, which);
+    return hipLaunchKernel(k, dim3(L.grid), dim3(L.threads), args, L.lds, st);
+  };
+  if (has_obstructions(B.T.ob) && !B.T.cull_all) {
+    const hipError_t e = go(TRACE_SOURCE_MAJOR_FREE);
+    if (e != hipSuccess) return e;
+  }
+  return go(TRACE_SOURCE_MAJOR);
+}
+
+hipError_t launch_torque_to_euler(const double* torque, const float* rotations, int nf,
+                                  double* g_rot, hipStream_t st) {
+  if (nf <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_torque_to_euler, dim3((nf + 255) / 256), dim3(256), 0, st, torque,
+                     rotations, nf, g_rot);
+  return hipGetLastError();
 }
 
 hipError_t launch_cull(const TraceArgs& A, int source_type, int all, float* cones,

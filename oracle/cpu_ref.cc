@@ -35,6 +35,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 #ifdef _OPENMP
@@ -552,9 +553,189 @@ int render_t(const iact_scene* sc, const double* sources, const float* values, i
   return 0;
 }
 
+/* ----------------------------------------------------------------- adjoint
+ * Gradient of L = sum_p G[p] image[p] (hard binning, primary-only scenes)
+ * with respect to the stage-0 rotations (Euler degrees) and the source
+ * values -- what jax.grad of render (render.py:174-220) gives for them:
+ * the pixel of a ray is piecewise constant, so only the ray value
+ * v = value (-d.n) / w * shadow (render.py:141) carries a derivative, through
+ * the world normal n = normalize(R (nl + s delta)) and, for point sources,
+ * the direction d = normalize(R pl + pos - src) (mirrors.py:64-79).  fp32
+ * arithmetic as the forward; facets are spread over the threads (each
+ * facet's gradient has one writer), per-sample adjoints are summed over the
+ * sources first and pushed through dR/dangle once per sample. */
+template <class R>
+void euler_grads(const float* a, R* D /* 3 x 9: dR/d(angle k), per radian */) {
+  const R k = (R)(3.14159265358979323846 / 180.0);
+  const R ra = (R)a[0] * k, rb = (R)a[1] * k, rc = (R)a[2] * k;
+  const R sa = std::sin(ra), ca = std::cos(ra), sb = std::sin(rb), cb = std::cos(rb);
+  const R sc = std::sin(rc), cc = std::cos(rc);
+  // R = Rz(c) Ry(b) Rx(a); columns of Rx: e0, (0, ca, sa), (0, -sa, ca)
+  const R Rzy[9] = {cc * cb, -sc, cc * sb, sc * cb, cc, sc * sb, -sb, (R)0, cb};
+  const R dRzy_b[9] = {-cc * sb, (R)0, cc * cb, -sc * sb, (R)0, sc * cb, -cb, (R)0, -sb};
+  const R dRzy_c[9] = {-sc * cb, -cc, -sc * sb, cc * cb, -sc, cc * sb, (R)0, (R)0, (R)0};
+  for (int i = 0; i < 3; ++i) {
+    // d/da: Rzy dRx/da, dRx/da columns: 0, (0, -sa, ca), (0, -ca, -sa)
+    D[0 * 9 + 3 * i + 0] = (R)0;
+    D[0 * 9 + 3 * i + 1] = -Rzy[3 * i + 1] * sa + Rzy[3 * i + 2] * ca;
+    D[0 * 9 + 3 * i + 2] = -Rzy[3 * i + 1] * ca - Rzy[3 * i + 2] * sa;
+    for (int j = 0; j < 2; ++j) {
+      const R* M = j == 0 ? dRzy_b : dRzy_c;
+      R* Dj = D + (1 + j) * 9;
+      Dj[3 * i + 0] = M[3 * i + 0];
+      Dj[3 * i + 1] = M[3 * i + 1] * ca + M[3 * i + 2] * sa;
+      Dj[3 * i + 2] = -M[3 * i + 1] * sa + M[3 * i + 2] * ca;
+    }
+  }
+}
+
+template <class R>
+int grad_t(const iact_scene* sc, const double* sources, const float* values, int ns,
+           int source_type, const double* G, double* g_rot, double* g_val, int nthreads) {
+  const iact_facets* Fc = &sc->facets;
+  const iact_sensor* S = &sc->sensor;
+  const int nf = Fc->n_facets, M = Fc->n_samples;
+  if (g_rot) memset(g_rot, 0, sizeof(double) * 3 * nf);
+  if (g_val) memset(g_val, 0, sizeof(double) * ns);
+  if (nf == 0 || ns == 0) return 0;
+  if (sc->n_stages > 0 || S->splat) return -2;
+  Frame<R> F;
+  F.pos = load3<R>(S->position);
+  F.u1 = V((R)S->rotation[0], (R)S->rotation[3], (R)S->rotation[6]);
+  F.u2 = V((R)S->rotation[1], (R)S->rotation[4], (R)S->rotation[7]);
+  F.n = V((R)S->rotation[2], (R)S->rotation[5], (R)S->rotation[8]);
+  F.ndotp = dot(F.n, F.pos);
+  F.gc = std::cos(-(R)S->grid_rotation);
+  F.gs = std::sin(-(R)S->grid_rotation);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  std::vector<double> gv_priv(g_val ? (size_t)nt * ns : 0, 0.0);
+#pragma omp parallel
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    std::vector<R> An((size_t)3 * M), Ap((size_t)3 * M), W((size_t)7 * M);
+    double* gvp = g_val ? gv_priv.data() + (size_t)tid * ns : nullptr;
+#pragma omp for schedule(dynamic, 1)
+    for (int f = 0; f < nf; ++f) {
+      R Rm[9];
+      euler<R>(Fc->rotations + 3 * f, Rm);
+      const v3<R> pos = load3<R>(Fc->positions + 3 * f);
+      for (int m = 0; m < M; ++m) {  // world samples (mirrors.py:64-79)
+        const size_t g = (size_t)f * M + m;
+        const v3<R> p = add(mul(Rm, load3<R>(Fc->points + 3 * g)), pos);
+        const v3<R> n = mul(Rm, load3<R>(Fc->normals + 3 * g));
+        const v3<R> dl = mul(Rm, load3<R>(Fc->delta + 3 * g));
+        const v3<R> pe = add(n, scl((R)Fc->perturbation_scale[f], dl));
+        const R nn = std::sqrt(dot(pe, pe));
+        R* w = W.data() + 7 * m;
+        w[0] = p.x, w[1] = p.y, w[2] = p.z;
+        w[3] = pe.x / nn, w[4] = pe.y / nn, w[5] = pe.z / nn;
+        w[6] = (R)Fc->weights[g];
+      }
+      std::fill(An.begin(), An.end(), (R)0);
+      std::fill(Ap.begin(), Ap.end(), (R)0);
+      for (int s = 0; s < ns; ++s) {
+        double gval = 0.0;
+        for (int m = 0; m < M; ++m) {
+          const R* w = W.data() + 7 * m;
+          const v3<R> p = V(w[0], w[1], w[2]), n = V(w[3], w[4], w[5]);
+          v3<R> d;
+          R inv_u = (R)0;  // 1 / |p - src| (point sources)
+          if (source_type == IACT_SOURCE_POINT) {
+            const double dx = (double)p.x - sources[3 * s], dy = (double)p.y - sources[3 * s + 1];
+            const double dz = (double)p.z - sources[3 * s + 2];
+            const double nr = sqrt(dx * dx + dy * dy + dz * dz);
+            d = V((R)(dx / nr), (R)(dy / nr), (R)(dz / nr));
+            inv_u = (R)(1.0 / nr);
+          } else {
+            d = V((R)sources[3 * s], (R)sources[3 * s + 1], (R)sources[3 * s + 2]);
+          }
+          const R sh = shadow<R>(&sc->obstructions, p, scl((R)-1, d));
+          if (sh == (R)0) continue;
+          const R c = dot(d, n);
+          const v3<R> r = sub(d, scl((R)2 * c, n));
+          const R ndotd = dot(r, F.n);
+          const bool par = std::fabs(ndotd) < (R)1e-10;
+          const R t = (F.ndotp - dot(p, F.n)) / (par ? (R)1 : ndotd);
+          if (par || !(t > (R)0)) continue;
+          const v3<R> op = sub(add(p, scl(t, r)), F.pos);
+          const R x = dot(op, F.u1), y = dot(op, F.u2);
+          const Bin b = S->kind == IACT_SENSOR_SQUARE ? bin_square<R, false>(S, x, y)
+                                                      : bin_hex<R, false>(S, &F, x, y);
+          if (b.pid < 0) continue;
+          // v = value * (-c) / w * shadow; dL/dv = G[pix]
+          const R gpix = (R)G[b.pid];
+          if (gvp) gval += (double)(gpix * (-c) / w[6] * sh);
+          const R k = gpix * (R)values[s] * sh / w[6];  // dL/d(-c)
+          // dL/dn = -k d; dL/dd = -k n, through d = normalize(p - src)
+          R* an = An.data() + 3 * m;
+          an[0] -= k * d.x, an[1] -= k * d.y, an[2] -= k * d.z;
+          if (source_type == IACT_SOURCE_POINT) {
+            const R dn = dot(d, n);
+            R* ap = Ap.data() + 3 * m;
+            ap[0] -= k * inv_u * (n.x - dn * d.x);
+            ap[1] -= k * inv_u * (n.y - dn * d.y);
+            ap[2] -= k * inv_u * (n.z - dn * d.z);
+          }
+        }
+        if (gvp) gvp[s] += gval;
+      }
+      if (!g_rot) continue;
+      // per sample: n = pe / |pe|, pe = R (nl + s delta); p = R pl + pos
+      R D[27];
+      euler_grads<R>(Fc->rotations + 3 * f, D);
+      double acc[3] = {0.0, 0.0, 0.0};
+      for (int m = 0; m < M; ++m) {
+        const size_t g = (size_t)f * M + m;
+        const R* w = W.data() + 7 * m;
+        const v3<R> n = V(w[3], w[4], w[5]);
+        const v3<R> nls = add(load3<R>(Fc->normals + 3 * g),
+                              scl((R)Fc->perturbation_scale[f], load3<R>(Fc->delta + 3 * g)));
+        const v3<R> pl = load3<R>(Fc->points + 3 * g);
+        const v3<R> pe = mul(Rm, nls);
+        const R ipn = (R)1 / std::sqrt(dot(pe, pe));
+        const v3<R> an = V(An[3 * m], An[3 * m + 1], An[3 * m + 2]);
+        const v3<R> ap = V(Ap[3 * m], Ap[3 * m + 1], Ap[3 * m + 2]);
+        // dL/dpe = (an - (an.n) n) / |pe|
+        const v3<R> gpe = scl(ipn, sub(an, scl(dot(an, n), n)));
+        for (int kk = 0; kk < 3; ++kk) {
+          const v3<R> dpe = mul(D + 9 * kk, nls), dp = mul(D + 9 * kk, pl);
+          acc[kk] += (double)(dot(gpe, dpe) + dot(ap, dp));
+        }
+      }
+      const double k = 3.14159265358979323846 / 180.0;
+      for (int kk = 0; kk < 3; ++kk) g_rot[3 * f + kk] = acc[kk] * k;
+    }
+  }
+  if (g_val)
+    for (int t = 0; t < nt; ++t)
+      for (int i = 0; i < ns; ++i) g_val[i] += gv_priv[(size_t)t * ns + i];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+/* dL/d(stage-0 rotations) (n_facets, 3) and dL/d(values) (n_sources) of
+ * L = sum_p G[p] image[p], hard binning, primary-only scenes; fp32 arithmetic
+ * with FP64 source directions.  Either output may be NULL.  -2: stage >= 1
+ * mirrors or a splat sensor (not covered). */
+int cpu_ref_grad(const iact_scene* sc, const double* sources, const float* values, int ns,
+                 int source_type, const double* grad_image, double* grad_rotations,
+                 double* grad_values, int nthreads) {
+  if (!sc || !grad_image || (ns > 0 && (!sources || !values))) return -1;
+  return grad_t<float>(sc, sources, values, ns, source_type, grad_image, grad_rotations,
+                       grad_values, nthreads);
+}
 
 /* precision 32: the reference's fp32 arithmetic (with FP64 source directions
  * and Newton); 64: FP64 throughout.  image: (n_pixels) double.  band_image

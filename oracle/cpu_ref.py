@@ -21,6 +21,10 @@ def lib():
                                      ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
                                      ctypes.c_void_p]
+        L.cpu_ref_grad.restype = ctypes.c_int
+        L.cpu_ref_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int]
         L.cpu_ref_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -52,6 +56,23 @@ def render_band(scene_struct, sources, values, source_type, n_pixels, nthreads=0
     if rc != 0:
         raise RuntimeError("cpu_ref_render failed")
     return out, bimg, {"binned": int(st3[0]), "band": int(st3[1]), "traced": int(st3[2])}
+
+
+def grad(scene_struct, sources, values, source_type, grad_image, n_facets, nthreads=0):
+    """(dL/drotations (n_facets, 3), dL/dvalues (n_sources,)) of
+    L = sum(grad_image * image), hard binning, primary-only (fp32 arithmetic)."""
+    src = np.ascontiguousarray(sources, np.float64)
+    val = np.ascontiguousarray(values, np.float32)
+    G = np.ascontiguousarray(grad_image, np.float64).reshape(-1)
+    g_rot = np.zeros((n_facets, 3), np.float64)
+    g_val = np.zeros(len(val), np.float64)
+    st = 0 if source_type == "point" else 1
+    rc = lib().cpu_ref_grad(ctypes.byref(scene_struct), src.ctypes.data, val.ctypes.data,
+                            len(val), st, G.ctypes.data, g_rot.ctypes.data, g_val.ctypes.data,
+                            int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"cpu_ref_grad failed ({rc})")
+    return g_rot, g_val
 
 
 def max_threads():

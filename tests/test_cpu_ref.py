@@ -91,3 +91,38 @@ def test_band_image_bounds_fp32_vs_fp64(case):
     # a band of zero width explains nothing; a planted offset of the fp32
     # image is caught
     assert not np.all(np.abs(i32 * (1 + 1e-3) - i64) <= b64 + slack)
+
+
+@pytest.mark.parametrize("case", ["hex-point-obstructed", "square-parallel"])
+def test_cpu_ref_grad_matches_twin(case):
+    """The CPU port's adjoint (rotation and value gradients of a weighted
+    hard-binned image) against float64 torch autograd of the oracle twin."""
+    import torch
+    import oracle.torch_twin as TW
+    if case == "hex-point-obstructed":
+        name, sc, src, vals, st = _scenes()[0]
+        sc.points, sc.normals, sc.delta, sc.weights = (a[:40] for a in (sc.points, sc.normals,
+                                                                         sc.delta, sc.weights))
+        sc.positions, sc.rotations, sc.scale = sc.positions[:40], sc.rotations[:40], sc.scale[:40]
+        rng = np.random.default_rng(3)
+        sc.scale = rng.uniform(0, 2e-3, 40).astype(np.float32)
+    else:
+        sensor = O.SquareSensorSpec((0, 0, 5.0), (0, 0, 0), 16, 16, (-0.5, 0.5, -0.5, 0.5))
+        sc = _single_facet_scene(O.Surface(0.1, -1.0), 1.0, sensor, m=2000,
+                                 rotation=(0.4, -0.3, 10.0))
+        src = np.array([[0.01, 0.0, -1.0], [-0.02, 0.01, -1.0]])
+        vals = np.array([1.0, 0.6], np.float32)
+        st = "parallel"
+    P = sc.sensor.n_pixels
+    W = 1.0 + 0.5 * np.sin(np.arange(P) * 0.37)
+    keep = []
+    s = cpu_ref.scene_struct(sc, keep)
+    g_rot, g_val = cpu_ref.grad(s, src, vals, st, W, sc.points.shape[0], nthreads=4)
+    rot = torch.tensor(sc.rotations, dtype=torch.float64, requires_grad=True)
+    v = torch.tensor(vals, dtype=torch.float64, requires_grad=True)
+    img = TW.render_image(sc, src, v, st, rot)
+    (img * torch.as_tensor(W)).sum().backward()
+    for label, a, b in (("rotations", g_rot, rot.grad.numpy()), ("values", g_val, v.grad.numpy())):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"[cpu grad {case}] d/d{label}: max rel err {err:.2e}")
+        assert np.abs(b).max() > 0 and err < 1e-3, (label, err)

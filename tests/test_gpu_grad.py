@@ -16,6 +16,7 @@ import pytest
 import torch
 
 import iactrace_amd as ia
+import oracle.iact_oracle as O
 import oracle.torch_twin as TW
 from iactrace_amd import configs
 from tests.helpers import band_mask, hex_kink_margin, oracle_scene
@@ -110,14 +111,24 @@ def test_grad_splat_hex_obstructions():
     gk, _ = _kernel_grads(tel, src, vals, "point", W)
     gt, _ = _twin_grads(tel, src, vals, "point", W)
     # a facet with a ray whose nearest hexagon (which fixes the splat's tap
-    # set), shadow decision or hex-norm branch (a kink of the tap weight) can
-    # flip between fp32 and fp64 carries an O(1/rays) difference; those facets
-    # are left out of the pose comparison
+    # set) or shadow decision the kernels take differently from the float64
+    # oracle (the twin's decisions), or whose hex-norm branch (a kink of the
+    # tap weight) fp32 may take differently, carries an O(1/rays) difference;
+    # those facets are left out of the pose comparison
     sc = oracle_scene(tel)
-    band, x, y, _ = band_mask(sc, src, vals, "point")
-    band |= hex_kink_margin(x, y, sc.sensor) < 2e-4
+    x, y, v64, _ = O.render(sc, src, vals, "point", "fp64", "debug")
+    xyk, vk = ia.render_debug(tel, torch.as_tensor(src), torch.as_tensor(vals), "point")
+    xyk, vk = xyk.cpu().numpy(), vk.cpu().numpy()
+    base = O.hex_grid_coords(np.asarray(x, np.float64), np.asarray(y, np.float64), sc.sensor)
+    kern = O.hex_grid_coords(xyk[:, 0].astype(np.float64), xyk[:, 1].astype(np.float64),
+                             sc.sensor)
+    to_ax = lambda g: O.axial_round(*O._to_axial(g[0], g[1], sc.sensor.hex_size,  # noqa: E731
+                                                  np.float64))
+    (q64, r64), (qk, rk) = to_ax(base), to_ax(kern)
+    differ = (q64 != qk) | (r64 != rk) | ((vk != 0) != (v64 != 0))
+    differ |= hex_kink_margin(x, y, sc.sensor) < 2e-4
     nf = sc.points.shape[0]
-    keep = ~band.reshape(nf, len(src), -1).any((1, 2))
+    keep = ~differ.reshape(nf, len(src), -1).any((1, 2))
     print(f"[splat-hex] facets compared {int(keep.sum())} of {nf}", flush=True)
     _compare("splat-hex", gk, gt, ["rot", "pos", "src", "val"], facet_mask=keep)
 
@@ -161,7 +172,7 @@ def test_grad_primary_aspheric(splat):
     # parallel rays on a hard sensor: moving a facet only moves its spot
     # across pixel boundaries, so the position gradient is exactly zero
     _compare(f"aspheric-{'splat' if splat else 'hard'}", gk, gt, ["asph", "rot", "pos"],
-             tol=2e-3, zero_keys=() if splat else ("pos",))
+             tol=1e-3, zero_keys=() if splat else ("pos",))
 
 
 def _facet(tel, f):
@@ -210,18 +221,24 @@ def _fd_check(label, tel, src, vals, stype, W, facets, h_rad, rtol):
 
 
 def test_grad_vs_central_differences_splat():
-    """Square Gaussian splat (sigma 0.7 px, 7x7 taps: sampling ripple and tap
-    truncation both below 1e-4) on the Davies-Cotton dish; no obstructions,
-    whose shadow edges would make the difference quotient jump."""
+    """Square Gaussian splat (sigma 0.7 px) on the Davies-Cotton dish; no
+    obstructions, whose shadow edges would make the difference quotient jump.
+    The reference's splat renormalises the weights over a window of
+    (2k+1)^2 taps around floor(x) (square.py:144-173): when a ray crosses a
+    pixel edge the window moves and the image jumps by the weight of the tap
+    that enters, ~1e-4 of a ray for k = 3, which biases a difference quotient
+    by ~1e-3 at any step size (tests/test_twin.py pins this on the float64
+    twin); with k = 4 (9 x 9 taps) the jump is ~1e-7 and the adjoint matches
+    central differences to 1e-3."""
     wl = configs.workload("C2", n_sources=2, n_samples=256)
     tel = wl.telescope(DEV, key=7)
     sensor = ia.DifferentiableSquareSensor([0, 0, 15.0], [0, 0, 0], 64, 64,
-                                           (-0.8, 0.8, -0.8, 0.8), sigma=0.7, kernel_size=3)
+                                           (-0.8, 0.8, -0.8, 0.8), sigma=0.7, kernel_size=4)
     tel = tel.replace_sensor(sensor, 0)
     src, vals = configs.point_source_field(2, 0.5, seed=1)
     yy, xx = np.mgrid[0:64, 0:64]
     W = (1.0 + 0.5 * np.sin(xx / 64 * 5.0) + 0.4 * np.cos(yy / 64 * 4.0)).reshape(-1)
-    _fd_check("fd-splat", tel, src, vals, "point", W, [0, 41, 190, 300, 377], 1e-4, 2e-3)
+    _fd_check("fd-splat", tel, src, vals, "point", W, [0, 41, 190, 300, 377], 1e-4, 1e-3)
 
 
 def test_grad_vs_central_differences_total_flux_hard():
@@ -233,7 +250,7 @@ def test_grad_vs_central_differences_total_flux_hard():
     big = ia.SquareSensor([0, 0, 15.0], [0, 0, 0], 16, 16, (-2.0, 2.0, -2.0, 2.0))
     tel = tel.replace_sensor(big, 0)
     _fd_check("fd-flux", tel, wl.sources, wl.values, "point", np.ones(big.n_pixels),
-              [0, 17, 200, 381], 1e-2, 2e-3)
+              [0, 17, 200, 381], 1e-2, 1e-3)
 
 
 def _cassegrain_tel(splat, m=512):
@@ -292,4 +309,87 @@ def test_grad_through_stage_two(splat):
     gt = {"rot": z(trot), "pos": z(tpos), "asph": z(ta0), "asph2": z(ta1), "src": z(ts),
           "val": z(tv)}
     keys = ["rot", "pos", "asph", "asph2", "src", "val"]
-    _compare(f"stage2-{'splat' if splat else 'hard'}", gk, gt, keys, tol=3e-3)
+    _compare(f"stage2-{'splat' if splat else 'hard'}", gk, gt, keys, tol=1e-3)
+
+
+def _grads_both_layouts(tel, src, vals, stype, W):
+    """Rotation, position, source and value gradients of sum(W image) from
+    the source-major adjoint (the default from 64 sources) and from the
+    sample-major one (layout 1)."""
+    out = []
+    for layout in (0, 1):
+        g = tel.mirror_groups[0]
+        rot = g.rotations.detach().clone().requires_grad_(True)
+        pos = g.positions.detach().clone().requires_grad_(True)
+        t2 = ia.Telescope([g.replace(rotations=rot, positions=pos)], tel.obstruction_groups,
+                          tel.sensors)
+        s = torch.tensor(src, dtype=torch.float64, device=DEV, requires_grad=True)
+        v = torch.tensor(vals, dtype=torch.float32, device=DEV, requires_grad=True)
+        img = ia.render(t2, s, v, stype, launch=ia.LaunchConfig(layout=layout))
+        (img.reshape(-1).double() * torch.as_tensor(W, device=DEV)).sum().backward()
+        out.append({"rot": rot.grad.double().cpu().numpy(), "pos": pos.grad.double().cpu().numpy(),
+                    "src": s.grad.double().cpu().numpy(), "val": v.grad.double().cpu().numpy()})
+    return out
+
+
+@pytest.mark.parametrize("case", ["hex-point-obstructed", "square-point", "square-parallel"])
+def test_source_major_adjoint(case):
+    """The source-major adjoint (lanes = sources over the sample records,
+    per-facet torques) against the sample-major adjoint and, at a size the
+    twin follows, against float64 autograd of the oracle twin."""
+    if case == "hex-point-obstructed":
+        # C3's camera, box and masts over the inner 91 facets of its dish
+        wl = configs.workload("C3", n_sources=150, n_samples=8)
+        w2 = configs.Workload("c3-inner", configs.parabolic_dish(n_facets=91), wl.obstructions,
+                              wl.sensor, 8, wl.sources, wl.values, "point",
+                              tilt_sigma_deg=wl.tilt_sigma_deg, tilt_seed=wl.tilt_seed)
+        tel = w2.telescope(DEV, key=5)
+        src, vals, stype = wl.sources, wl.values, "point"
+    elif case == "square-point":
+        wl = configs.workload("C2", n_sources=100, n_samples=16)
+        tel = wl.telescope(DEV, key=6)
+        src, vals, stype = wl.sources, wl.values, "point"
+    else:
+        wl = configs.workload("C1", n_samples=16)
+        tel = wl.telescope(DEV, key=7)
+        src = configs.parallel_grid(9, 8, 0.25)
+        vals = np.linspace(0.5, 1.5, len(src)).astype(np.float32)
+        stype = "parallel"
+    W = _smooth_weights(tel.sensors[0])
+    sm, smaj = _grads_both_layouts(tel, src, vals, stype, W)
+    for k in ("rot", "pos", "src", "val"):
+        a, b = sm[k], smaj[k]
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / max(scale, 1e-300)
+        print(f"[adjoint {case}] d/d{k}: source-major vs sample-major max rel err {err:.2e} "
+              f"(scale {scale:.3g})", flush=True)
+        if stype == "parallel" and k in ("pos", "src"):
+            # parallel rays on a hard sensor: position gradients vanish
+            # (the sample-major kernel's tiny residue aside)
+            assert np.abs(a).max() <= 1e-6 * np.abs(sm["rot"]).max() + 1e-30 or err < 1e-4
+            continue
+        assert scale > 0 and err < 1e-4, (k, err)
+    gt, _ = _twin_grads(tel, src, vals, stype, W)
+    keys = ["rot", "val"] if stype == "parallel" else ["rot", "pos", "src", "val"]
+    # the twin takes its pixel and shadow decisions from the float64 oracle:
+    # a facet holding a ray that the kernels decide differently (an edge-band
+    # ray, printed by the forward parity tests) differs from the twin by that
+    # ray's weight jump, and is left out of the pose comparison
+    sc = oracle_scene(tel)
+    _, vk, pk = ia.render_debug(tel, torch.as_tensor(src), torch.as_tensor(vals), stype,
+                                return_pixels=True)
+    vk, pk = vk.cpu().numpy(), pk.cpu().numpy()
+    _, _, v64, p64 = O.render(sc, src, vals, stype, "fp64", "debug")
+    differ = (pk != p64) | ((vk != 0) != (v64 != 0))
+    nf = sc.points.shape[0]
+    keep = ~differ.reshape(nf, len(src), -1).any((1, 2))
+    print(f"[source-major {case}] facets compared {int(keep.sum())} of {nf}", flush=True)
+    # on a hard sensor d/dval_s = sum over s's rays of W[pixel] * value / val_s
+    # exactly, so the differently decided rays shift the twin's value
+    # gradient by a known amount
+    def wsum(p, v):
+        return np.where(p >= 0, W[np.clip(p, 0, None)], 0.0) * v
+    shift = np.where(differ, wsum(pk, vk.astype(np.float64)) - wsum(p64, v64), 0.0)
+    gt["val"] = gt["val"] + shift.reshape(nf, len(src), -1).sum((0, 2)) / np.asarray(vals,
+                                                                                    np.float64)
+    _compare(f"source-major {case}", sm, gt, keys, tol=1e-3, facet_mask=keep)

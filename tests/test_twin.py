@@ -78,3 +78,46 @@ def test_twin_gradients_match_its_finite_differences():
     assert np.allclose(src.grad.numpy(), fs, rtol=1e-4, atol=1e-6 * np.abs(fs).max())
     fa = _fd(lambda x: loss(asph=x), np.asarray(surf.aspheric, np.float64), 1e-9)
     assert np.allclose(a.grad.numpy(), fa, rtol=1e-4, atol=1e-6 * np.abs(fa).max())
+
+
+def test_splat_window_truncation_biases_differences():
+    """The reference's square splat renormalises its weights over the
+    (2k+1)^2 taps around floor(x) (square.py:144-173), so the image jumps by
+    the entering tap's weight whenever a ray crosses a pixel edge.  A central
+    difference of a tilt then carries a bias that does not shrink with the
+    step: ~1e-3 for sigma 0.7 px and k = 3, ~1e-6 for k = 4, on the float64
+    twin itself.  (Its autograd, like the kernels' adjoint, is the derivative
+    of the smooth part.)"""
+    import math
+    import torch
+    import iactrace_amd as ia
+    from iactrace_amd import configs
+    from tests.helpers import cpu_scene, oracle_sensor
+    wl = configs.workload("C2", n_sources=2, n_samples=256)
+    wl.mirrors = configs.davies_cotton_dish(n_facets=1)
+    src, vals = configs.point_source_field(2, 0.5, seed=1)
+    yy, xx = np.mgrid[0:64, 0:64]
+    W = torch.as_tensor((1.0 + 0.5 * np.sin(xx / 64 * 5.0) + 0.4 * np.cos(yy / 64 * 4.0))
+                        .reshape(-1))
+    rel = {}
+    for k in (3, 4):
+        sc = cpu_scene(wl, key=7)
+        sc.sensor = oracle_sensor(ia.DifferentiableSquareSensor(
+            [0, 0, 15.0], [0, 0, 0], 64, 64, (-0.8, 0.8, -0.8, 0.8), sigma=0.7, kernel_size=k))
+        rot0 = torch.tensor(sc.rotations, dtype=torch.float64)
+
+        def loss(rot):
+            return (TW.render_image(sc, src, vals, "point", rot) * W).sum()
+        r = rot0.clone().requires_grad_(True)
+        loss(r).backward()
+        ad = r.grad[0, 0].item()
+
+        def cd(h):
+            rp, rm = rot0.clone(), rot0.clone()
+            rp[0, 0] += h
+            rm[0, 0] -= h
+            return (loss(rp) - loss(rm)).item() / (2 * h)
+        h = math.degrees(1e-4)
+        fd = (4 * cd(h / 2) - cd(h)) / 3
+        rel[k] = abs(fd - ad) / abs(fd)
+    assert rel[3] > 2e-4 and rel[4] < 1e-5, rel
