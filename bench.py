@@ -26,6 +26,7 @@ Also reported:
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-parity]
                        [--extras C2,C3,C3T,C4 | none] [--c5-instances 64]
+                       [--only C2|C3|C3T|C4|copy]
 """
 from __future__ import annotations
 
@@ -284,6 +285,19 @@ def cpu_fwd_bwd(bw, tel, rot, targets, reps=5):
 
 
 # ------------------------------------------------------------------ extras
+def hbm_copy(steps, warmup, nbytes=1 << 30):
+    """A plain device copy of `nbytes` (the calibration of the HBM byte
+    counters: FETCH_SIZE and WRITE_SIZE of this kernel are nbytes each)."""
+    import numpy as np
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    _, ev = time_steps(lambda: b.copy_(a), steps, warmup)
+    ms = float(np.mean(ev))
+    return {"workload": f"device copy of {nbytes} bytes", "ms": ms,
+            "gb_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
 def c2_forward(ia, configs, steps, warmup, want_parity):
     import numpy as np
     import torch
@@ -413,6 +427,10 @@ def main():
     ap.add_argument("--extras", default="C2,C3,C3T,C4",
                     help="comma list of extra lines (C2, C3, C3T, C4) or 'none'")
     ap.add_argument("--c5-instances", type=int, default=64)
+    ap.add_argument("--only", default=None,
+                    help="run one extra (C2, C3, C3T, C4, or copy: a 1 GiB device copy) "
+                         "alone and print its record instead of the headline line (for "
+                         "profiling one workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -428,6 +446,16 @@ def main():
     import iactrace_amd as ia
     from iactrace_amd import configs
 
+    if args.only:
+        fns = {"copy": lambda: hbm_copy(args.steps, args.warmup),
+               "C2": lambda: c2_forward(ia, configs, args.steps, args.warmup, False),
+               "C3": lambda: c3_fwd_bwd(ia, configs, args.steps, args.warmup),
+               "C3T": lambda: c3t_culling(ia, configs, args.steps, args.warmup),
+               "C4": lambda: c4_response(ia, configs, args.steps, args.warmup)}
+        if args.only not in fns:
+            sys.exit(f"unknown workload for --only: {args.only}")
+        print(json.dumps({"only": args.only, **fns[args.only]()}), flush=True)
+        return
     r = c5(ia, configs, args.c5_instances, args.steps, args.warmup, not args.no_parity,
            not args.no_plain)
     bw, tel, rot, targets = r.pop("_state")

@@ -51,7 +51,8 @@ class IactFacets(Structure):
                 ("points", c_void_p), ("normals", c_void_p), ("delta", c_void_p),
                 ("weights", c_void_p), ("positions", c_void_p), ("rotations", c_void_p),
                 ("perturbation_scale", c_void_p), ("surface_index", c_void_p),
-                ("offsets", c_void_p), ("area_over_m", c_void_p)]
+                ("offsets", c_void_p), ("area_over_m", c_void_p),
+                ("max_sample_radius", c_float)]
 
 
 class IactObstructions(Structure):

@@ -244,6 +244,9 @@ void fill_trace_args(TraceArgs& A, const iact_scene& sc, const double* sources,
   A.sources = sources;
   A.values = values;
   A.ns = ns;
+  // far point sources: |e| >= 2^30 max(R, 1 mm), R the samples' radius bound
+  const double rf = (double)fmaxf(F.max_sample_radius, 1e-3f) * 1073741824.0;
+  A.far_e2 = F.max_sample_radius > 0.f ? (float)(rf * rf) : INFINITY;
   A.n_mirrors = sc.n_mirrors;
   A.n_stages = sc.n_stages;
   for (int i = 0; i < sc.n_stages; ++i) A.stages[i] = sc.stages[i];

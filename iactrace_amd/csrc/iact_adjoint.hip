@@ -345,10 +345,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
         set_facet(smp, facet_cull(A, f, chunk));
       }
       const f3 e = lane_value(e_lane, k);
+      const float qs = source_qs(A, e);
       const float value = vals[s];
       // ---- forward recompute (the operations of trace_ray)
       float inv;
-      const f3 d = primary_direction<SRC>(smp, e, inv);
+      const f3 d = primary_direction<SRC>(smp, e, qs, inv);
       const float c = dot(d, smp.n);
       const f3 r = fma3(-2.f * c, smp.n, d);
       const float shadow = primary_shadow<2>(A, smp, d, active);
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
         if (SRC == IACT_SOURCE_POINT && (!STAGES || A.n_stages == 0)) {
           // the hit as trace_ray forms it (the reflection of u = q + e, not
           // normalised), so that the pixel is the forward's
-          const f3 u = smp.q + e;
+          const f3 u = fma3(qs, smp.q, e);
           const f3 ru = fma3(-2.f * dot(u, smp.n), smp.n, u);
           ok = plane_grid<KIND, AXIS>(A.S, Kg, ru, gu_, gv_, inv) && active;
         } else {

@@ -562,7 +562,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
       const float nv = vals[s + 1 < s1 ? s + 1 : s];
       float gu, gv, v, x, y;
       const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, 2, MODE == IACT_MODE_DEBUG>(
-          A, smp, active, e, sv, gu, gv, v, x, y);
+          A, smp, active, e, source_qs(A, e), sv, gu, gv, v, x, y);
       sv = nv;
       if (MODE == IACT_MODE_DEBUG) {
         if (active) {
@@ -801,6 +801,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     // their refinement for the sample block, and each lane source's offset
     FacetCull fc[SPL];
     f3 e[SPL];
+    float qs[SPL];
     bool refined = false;
     int rlist[SPL];
     uint64_t rbits[SPL];
@@ -813,10 +814,12 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     // kernel forms the reflection from n (its records' m words stay unread:
     // its scalar registers are the scarcer resource).  GEN (generic) false:
     // no parallel-ray test, no edge test and square pixels (the launch needs
-    // neither test and has such pixels).
-    auto trace_one = [&](auto sh, auto gen, const RecRaw& rec) {
+    // neither test and has such pixels).  FR: every lane's sources are far
+    // (source_qs 0), u = e.
+    auto trace_one = [&](auto sh, auto gen, auto far, const RecRaw& rec) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
       constexpr bool GEN = decltype(gen)::value;
+      constexpr bool FR = decltype(far)::value;
       const Sample smp0 = unpack_record(rec);
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
@@ -830,8 +833,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
-        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0, GEN>(
-            A, smp, true, e[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
+        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0, GEN, FR>(
+            A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
         if (!SPLAT) {
           // run of equal pixels per lane and source (consecutive samples of
           // a facet put a source's rays on the same one or two pixels), added
@@ -883,29 +886,42 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         refined = refine_candidates<SPL>(A, f, c, j >> 6, lane, fc, rlist, rbits);
       const cptr<float> fp = as_const(A.positions) + 3 * f;  // scalar loads
       const f3 pos = mk3(fp[0], fp[1], fp[2]);
+      int near = 0;
 #pragma unroll
-      for (int k = 0; k < SPL; ++k) e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+      for (int k = 0; k < SPL; ++k) {
+        e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+        qs[k] = source_qs(A, e[k]);
+        near |= qs[k] != 0.f;
+      }
+      const bool all_far = SRC == IACT_SOURCE_POINT && __ballot(near) == 0ull;
       // the next record is loaded while one is traced (the record table has
       // padding at its end, so the prefetch needs no clamping)
       const float4* rp = recs + 4 * (size_t)j;
       RecRaw cur = fetch_record(rp);
-      auto run = [&](auto sh, auto gen) {
+      auto run = [&](auto sh, auto gen, auto far) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
-          trace_one(sh, gen, cur);
+          trace_one(sh, gen, far, cur);
           cur = nxt;
         }
       };
+      using T_ = std::true_type;
+      using F_ = std::false_type;
       if (OBST && PART == 2) {
-        // (one loop: the shadow tests dominate, and a second copy would cost
-        // this kernel scalar registers)
-        run(std::true_type{}, std::true_type{});
-      } else {
-        if (lean)
-          run(std::false_type{}, std::false_type{});
+        // (no generic copy: the shadow tests dominate, and more copies would
+        // cost this kernel scalar registers)
+        if (all_far)
+          run(T_{}, T_{}, T_{});
         else
-          run(std::false_type{}, std::true_type{});
+          run(T_{}, T_{}, F_{});
+      } else if (lean) {
+        if (all_far)
+          run(F_{}, F_{}, T_{});
+        else
+          run(F_{}, F_{}, F_{});
+      } else {
+        run(F_{}, T_{}, F_{});
       }
     }
     if (!SPLAT) {
@@ -987,14 +1003,28 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
     const float4* recs = (const float4*)A.recs;
     FacetCull fc[SPL];
     f3 e[SPL];
+    float qs[SPL];
     bool refined = false;
     int rlist[SPL];
     uint64_t rbits[SPL];
+    // far sources (source_qs 0 on every lane): each source's u = e, d and
+    // 1/|u| are fixed over a facet run, and the lane sums per source
+    //   N = sum b n,  C = sum b c2     (b = dc, c2 = -2 d.n)
+    // over the run, from which its dL/dp is inv (N + C d / 2) and its torque
+    // share N x d.  The torque's other terms, inv (sum b q x n + sum b c2 q x
+    // d / 2), are below 2^-30 * 1.5 * sum |b| (|q| inv <= 2^-30, |c2| <= 2):
+    // some 40 times below the fp32 rounding of the sum N x d that they join
+    // (also per ray, in the reference's fp32 gradient), and are left out.
+    f3 fd[SPL], fN[SPL];
+    float finv[SPL], fC[SPL];
+    const bool want_pc = want_pos || want_src;
     // one sample's rays (SPL per lane): the forward's trace up to the pixel,
-    // then the lane's share of the sample's torque and dL/dp (tq, gp)
-    auto adjoint_one = [&](auto sh, auto gen, const RecRaw& rec, f3& tq, f3& gp) {
+    // then the lane's share of the sample's torque and dL/dp (tq, gp), or
+    // (FR) the far sums
+    auto adjoint_one = [&](auto sh, auto gen, auto far, const RecRaw& rec, f3& tq, f3& gp) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
       constexpr bool GEN = decltype(gen)::value;
+      constexpr bool FR = decltype(far)::value;
       constexpr bool RM = OB == 0;
       const Sample smp0 = unpack_record(rec);
       float san_x = 0.f, san_y = 0.f, san_z = 0.f, st = 0.f;
@@ -1012,14 +1042,20 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
         float inv, c2u, rs;
         f3 u, d, r;
         if (SRC == IACT_SOURCE_POINT) {
-          u = smp.q + e[k];
-          inv = frsq(dot(u, u));
+          if (FR) {
+            u = e[k];
+            inv = finv[k];
+            d = fd[k];
+          } else {
+            u = fma3(qs[k], smp.q, e[k]);
+            inv = frsq(dot(u, u));
+            d = inv * u;
+          }
           c2u = RM ? dot(u, smp.m) : -2.f * dot(u, smp.n);
           r = fma3(c2u, smp.n, u);
           rs = inv;
-          d = inv * u;
         } else {
-          d = primary_direction<SRC>(smp, e[k], inv);
+          d = primary_direction<SRC>(smp, e[k], 1.f, inv);
           c2u = RM ? dot(d, smp.m) : -2.f * dot(d, smp.n);
           r = fma3(c2u, smp.n, d);
           rs = 1.f;
@@ -1037,7 +1073,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
           const float gs = OB ? gp_pix * shadow : gp_pix;
           gval[k] = fmaf(gs * (c2u * inv), smp.nrwh, gval[k]);
         }
-        if (SRC == IACT_SOURCE_POINT) {
+        if (SRC == IACT_SOURCE_POINT && FR) {
+          const float b = a * smp.nrw;
+          fN[k] = fma3(b, smp.n, fN[k]);
+          if (want_pc) fC[k] = fmaf(b, c2u * inv, fC[k]);
+        } else if (SRC == IACT_SOURCE_POINT) {
           const float k1 = a * inv;  // dc / |u| before nrw
           san_x = fmaf(k1, u.x, san_x);
           san_y = fmaf(k1, u.y, san_y);
@@ -1065,6 +1105,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
           }
         }
       }
+      if (FR) return;
       // the lane's share of the sample's torque: linear in the lane's sums,
       // so the wave reduces only once per facet run
       const float nrw = smp0.nrw;
@@ -1098,29 +1139,60 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
         refined = refine_candidates<SPL>(A, f, c, j >> 6, lane, fc, rlist, rbits);
       const cptr<float> fp = as_const(A.positions) + 3 * f;
       const f3 pos = mk3(fp[0], fp[1], fp[2]);
+      int near = 0;
 #pragma unroll
-      for (int k = 0; k < SPL; ++k) e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+      for (int k = 0; k < SPL; ++k) {
+        e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+        qs[k] = source_qs(A, e[k]);
+        near |= qs[k] != 0.f;
+        // u = e: trace_ray's 1/|u| and d of every far sample
+        finv[k] = frsq(dot(e[k], e[k]));
+        fd[k] = finv[k] * e[k];
+        fN[k] = mk3(0.f, 0.f, 0.f);
+        fC[k] = 0.f;
+      }
+      const bool all_far = SRC == IACT_SOURCE_POINT && __ballot(near) == 0ull;
       const float4* rp = recs + 4 * (size_t)j;
       RecRaw cur = fetch_record(rp);
       f3 T = mk3(0.f, 0.f, 0.f), Gp = mk3(0.f, 0.f, 0.f);
-      auto run = [&](auto sh, auto gen) {
+      auto run = [&](auto sh, auto gen, auto far) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
           f3 tq, gp;
-          adjoint_one(sh, gen, cur, tq, gp);
-          T = T + tq;
-          Gp = Gp + gp;
+          adjoint_one(sh, gen, far, cur, tq, gp);
+          if (!decltype(far)::value) {
+            T = T + tq;
+            Gp = Gp + gp;
+          }
           cur = nxt;
         }
       };
+      using T_ = std::true_type;
+      using F_ = std::false_type;
       if (OBST && PART == 2) {
-        run(std::true_type{}, std::true_type{});
-      } else {
-        if (lean)
-          run(std::false_type{}, std::false_type{});
+        if (all_far)
+          run(T_{}, T_{}, T_{});
         else
-          run(std::false_type{}, std::true_type{});
+          run(T_{}, T_{}, F_{});
+      } else if (lean) {
+        if (all_far)
+          run(F_{}, F_{}, T_{});
+        else
+          run(F_{}, F_{}, F_{});
+      } else {
+        run(F_{}, T_{}, F_{});
+      }
+      if (all_far) {
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+          T = T + cross(fN[k], fd[k]);
+          if (want_pc) {
+            const f3 gpk = finv[k] * fma3(0.5f * fC[k], fd[k], fN[k]);
+            Gp = Gp + gpk;
+            if (want_src) gsrc[k] = gsrc[k] - gpk;
+          }
+        }
       }
       T = mk3(wave_sum(T.x), wave_sum(T.y), wave_sum(T.z));
       if (want_pos) Gp = mk3(wave_sum(Gp.x), wave_sum(Gp.y), wave_sum(Gp.z));
@@ -1194,6 +1266,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
     const float val = A.values[s];
     int fe = -1;
     f3 e = mk3(0.f, 0.f, 0.f);
+    float qs = 1.f;
     for (long long c = wave; c < nchunks; c += wpb) {
       const int f = (int)fdiv((uint32_t)c, A.d_cpf);
       const int m = ((int)c - f * A.cpf) * 64 + lane;
@@ -1204,12 +1277,14 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
       const f3 pos = mk3(A.positions[3 * f], A.positions[3 * f + 1], A.positions[3 * f + 2]);
       if (f != fe) {
         e = source_offset<SRC>(sx, sy, sz, pos);
+        qs = source_qs(A, e);
         fe = f;
       }
       Sample smp = make_sample<KIND>(A.S, p, n, A.world[6 * total + gi], pos);
       set_facet(smp, facet_cull(A, f, s >> 6));
       float gu, gv, v, x, y;
-      const bool ok = trace_ray<SRC, KIND, true, false>(A, smp, active, e, val, gu, gv, v, x, y);
+      const bool ok =
+          trace_ray<SRC, KIND, true, false>(A, smp, active, e, qs, val, gu, gv, v, x, y);
       if (!SPLAT) {
         const int pix = grid_pixel<KIND>(A.S, table, gu, gv, ok);
         if (A.lds_image)

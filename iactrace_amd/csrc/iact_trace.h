@@ -61,6 +61,9 @@ struct TraceArgs {
   const double* sources;
   const float* values;
   int ns;
+  // far point sources: |e|^2 from which a (facet, source) pair forms its
+  // direction from e alone (+inf: never)
+  float far_e2;
   int source_tile, n_stiles;
   long long n_items;
   // stage >= 1
@@ -326,10 +329,19 @@ IACT_DEV f3 source_offset(double sx, double sy, double sz, f3 facet_pos) {
   return mk3((float)sx, (float)sy, (float)sz);
 }
 
+// A point source so far from a facet that the directions from its samples
+// differ from the one from the facet position by < 2^-30 rad (|q| <= R for
+// every sample and |e| >= 2^30 R; fp32 resolves 2^-24) takes u = e for every
+// sample: one direction per (facet, source), as the reference's fp32
+// p - source (render.py:130), which loses p at such distances, has it.
+// Returns qs, the multiplier of q in u = qs q + e: 0 far, 1 otherwise.  The
+// same test in every kernel keeps their rays alike.
+IACT_DEV float source_qs(const TraceArgs& A, f3 e) { return dot(e, e) >= A.far_e2 ? 0.f : 1.f; }
+
 template <int SRC>
-IACT_DEV f3 primary_direction(const Sample& s, f3 e, float& inv) {
+IACT_DEV f3 primary_direction(const Sample& s, f3 e, float qs, float& inv) {
   if (SRC == IACT_SOURCE_POINT) {
-    const f3 u = s.q + e;
+    const f3 u = fma3(qs, s.q, e);
     inv = frsq(fmaf(u.x, u.x, fmaf(u.y, u.y, u.z * u.z)));
     return mk3(u.x * inv, u.y * inv, u.z * inv);
   }
@@ -546,15 +558,15 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 // plane_test_redundant, on square pixels) leaves out the parallel-ray test
 // and the pixel aspect ratio; ok may then hold for a ray that lands far off
 // the grid, which binning drops.
-// e is the facet's source offset (source_offset).  Returns false when the
-// ray misses the plane (the reference's 1e10 sentinel) or the lane is
-// inactive; v is 0 for inactive lanes.  (gu, gv) is the hit in sensor-grid
-// coordinates (what the binning reads); XY also gives the hit (x, y) in the
-// sensor frame.
+// e is the facet's source offset (source_offset), qs its source_qs.
+// Returns false when the ray misses the plane (the reference's 1e10
+// sentinel) or the lane is inactive; v is 0 for inactive lanes.  (gu, gv) is
+// the hit in sensor-grid coordinates (what the binning reads); XY also gives
+// the hit (x, y) in the sensor frame.
 template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false,
-          bool RM = (OBST == 0), bool GEN = true>
-IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float value,
-                        float& gu, float& gv, float& v, float& x, float& y,
+          bool RM = (OBST == 0), bool GEN = true, bool FAR = false>
+IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float qs,
+                        float value, float& gu, float& gv, float& v, float& x, float& y,
                         float* vfac = nullptr) {
   // With m = -2 n, d.m = -2 (d.n) and value * (d.m) * nrwh = value * (d.n) *
   // nrw, both exactly (scalings by powers of two).  RM false forms d.m from
@@ -568,8 +580,9 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
     // The reflection of u = q + e, r = u + (u.m) n, is |u| times the unit
     // reflection, and the sensor hit depends only on its direction: u is not
     // normalised, 1/|u| enters the cosine and the parallel-ray test only
-    // (the unit direction d is formed for the shadow test alone).
-    const f3 u = s.q + e;
+    // (the unit direction d is formed for the shadow test alone).  FAR: the
+    // caller knows qs = 0 for every lane (u = e, loop-invariant).
+    const f3 u = FAR ? e : fma3(qs, s.q, e);
     inv = frsq(dot(u, u));
     const float c2u = RM ? dot(u, s.m) : -2.f * dot(u, s.n);
     r = fma3(c2u, s.n, u);
@@ -577,7 +590,7 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
     rs = inv;
     d = inv * u;
   } else {
-    d = primary_direction<SRC>(s, e, inv);
+    d = primary_direction<SRC>(s, e, qs, inv);
     c2 = RM ? dot(d, s.m) : -2.f * dot(d, s.n);
     r = fma3(c2, s.n, d);
   }

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 from collections import defaultdict
 
 import numpy as np
@@ -84,6 +85,24 @@ def _cat(ts, device):
     return ts[0] if len(ts) == 1 else torch.cat(ts, 0).contiguous()
 
 
+_RADII = {}
+
+
+def sample_radius(points):
+    """max |point| over a group's samples (facet frame), once per version of
+    the points tensor (one device read; later calls are free)."""
+    key = id(points)
+    hit = _RADII.get(key)
+    if hit is not None and hit[0]() is points and hit[1] == points._version:
+        return hit[2]
+    r = float(points.detach().float().norm(dim=-1).max()) if points.numel() else 0.0
+    if len(_RADII) > 256:
+        for k in [k for k, v in _RADII.items() if v[0]() is None]:
+            del _RADII[k]
+    _RADII[key] = (weakref.ref(points), points._version, r)
+    return r
+
+
 def facets_abi(groups, keep, rotations=None, positions=None):
     """IactFacets over the concatenation of stage-0 ``groups`` (render.py:201-205)."""
     dev = groups[0].positions.device
@@ -105,6 +124,9 @@ def facets_abi(groups, keep, rotations=None, positions=None):
         "perturbation_scale": _cat([g.perturbation_scale.reshape(-1) for g in groups], dev),
         "offsets": _cat([g.offsets for g in groups], dev),
     }
+    # the far-source test of the kernels bounds every sample's offset from
+    # its facet position by this radius
+    F.max_sample_radius = max(sample_radius(g.points) for g in groups)
     sidx = torch.cat([torch.full((len(g),), i, dtype=torch.int32) for i, g in enumerate(groups)])
     arrs["surface_index"] = sidx.to(dev)
     for k, v in arrs.items():

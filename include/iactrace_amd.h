@@ -78,6 +78,14 @@ typedef struct iact_facets {
                                       surface; NULL ok as above              */
   const float* area_over_m;        /* (n_facets) aperture area / n_samples;
                                       NULL ok as above                       */
+  float max_sample_radius;         /* an upper bound of |point| over every
+                                      sample, metres; 0 = not given.  A point
+                                      source farther than 2^30 times
+                                      max(this, 1 mm) from a facet takes one
+                                      direction per (facet, source), formed
+                                      from the facet position (the samples'
+                                      own differ from it by < 2^-30 rad);
+                                      otherwise, and when 0, one per ray    */
 } iact_facets;
 
 /* Obstruction tables, one per primitive type, in the reference's group order
