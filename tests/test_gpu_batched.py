@@ -195,3 +195,64 @@ def test_batched_plan_validation():
     with pytest.raises(ia._lib.IactError):
         ia.render_batched(tel, rot[:2].contiguous(), s, v, off, launch=ia.LaunchConfig(
             deterministic=True))
+
+
+def test_c5_reduced_instances_against_oracle():
+    """A reduced C5 -- 4 instances with their own tilt draws, 4 samples, 64
+    point sources each (the source-major kernels): every batched image
+    equals the plain render of its instance's telescope, and instances 0
+    and 3 meet the oracle per ray (pixel ids outside the edge band, values,
+    image; tests.test_gpu_forward)."""
+    from tests.test_gpu_forward import _trace_and_compare
+    bw = configs.batch_workload(4, n_min=64, n_max=64, n_samples=4)
+    tel = bw.telescope(DEV, key=3)
+    rot = bw.rotations(tel)
+    imgs = ia.render_batched(tel, rot, bw.sources, bw.values, bw.offsets, "point")
+    for i in range(4):
+        t = tel.set_mirror_rotations(0, rot[i])
+        a, b = int(bw.offsets[i]), int(bw.offsets[i + 1])
+        ref = ia.render(t, torch.as_tensor(bw.sources[a:b]), torch.as_tensor(bw.values[a:b]),
+                        "point").reshape(-1).double()
+        got = imgs[i].double()
+        sig = ref > 1e-3 * ref.max()
+        rel = ((got - ref).abs()[sig] / ref[sig]).max().item()
+        print(f"[C5 reduced instance {i}] batched vs plain max rel {rel:.2e}", flush=True)
+        assert rel < 1e-6
+        assert (got - ref).abs()[~sig].max().item() <= 1e-6 * ref.max().item()
+        if i in (0, 3):
+            _trace_and_compare(t, bw.sources[a:b], bw.values[a:b], "point",
+                               f"C5 reduced instance {i}")
+
+
+def test_c5_full_size_instances_and_fit():
+    """Full C5 (64 instances, 875 facets x 1024 samples, 548694 sources):
+    batched instances 0, 17 and 63 equal plain renders of their telescopes,
+    and the loss of the captured fit iteration falls over three steps."""
+    bw = configs.batch_workload(64)
+    tel = bw.telescope(DEV, key=0)
+    rot = bw.rotations(tel)
+    bp = BatchedPlan(tel, bw.sources, bw.values, bw.offsets, "point")
+    with torch.no_grad():
+        imgs = bp.forward(rot)
+    src = torch.as_tensor(bw.sources, device=DEV)
+    val = torch.as_tensor(bw.values, device=DEV)
+    for i in (0, 17, 63):
+        a, b = int(bw.offsets[i]), int(bw.offsets[i + 1])
+        ref = ia.render(tel.set_mirror_rotations(0, rot[i]), src[a:b], val[a:b], "point")
+        ref = ref.reshape(-1).double()
+        got = imgs[i].double()
+        sig = ref > 1e-3 * ref.max()
+        rel = ((got - ref).abs()[sig] / ref[sig]).max().item()
+        print(f"[C5 instance {i}] {b - a} sources, max rel vs plain render {rel:.2e}", flush=True)
+        assert rel < 1e-6
+    base = tel.mirror_groups[0].rotations.detach()
+    with torch.no_grad():
+        targets = bp.forward(base.unsqueeze(0).repeat(64, 1, 1).contiguous())
+    fit = AlignmentFit(tel, rot, bw.sources, bw.values, bw.offsets, targets, lr=1e-4)
+    losses = []
+    for _ in range(4):
+        fit.step()
+        losses.append(float(fit.loss.sum()))
+    fit.close()
+    print(f"[C5 fit] losses {losses}", flush=True)
+    assert losses[-1] < losses[0]
