@@ -103,7 +103,7 @@ def image_parity(sc, src, vals, source_type, gpu_image, label):
     relative L1 error, the largest relative error of a significant pixel
     (>= 1e-3 of the peak) and the same after taking away the pixel's
     edge-band mass (|value| of the FP64 rays within the edge band of a
-    binning boundary -- 1e-3 px or 2.5e-5 m, tests.helpers.band_units -- at
+    binning boundary -- 2.5e-5 m on the sensor, tests.helpers.band_units -- at
     their pixel and the one across the nearest edge: the rays an fp32
     coordinate may put on either side); and the fp32 CPU port's L1 error,
     the fp32 floor that any evaluation of the reference's arithmetic has."""

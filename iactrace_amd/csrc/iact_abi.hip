@@ -300,6 +300,16 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
   if (ts > ns) ts = ns > 0 ? ns : 1;
   // items of one instance (chunks x source tiles) are decoded in 32 bits
   const long long nch = (long long)A.nf * A.cpf;
+  if (!(L && L->source_tile) && nch > 0 && ns > 0) {
+    // a scene of few sample chunks (few facets): smaller source tiles (down
+    // to 16 sources), so that the items give every SIMD eight waves
+    const long long want = 8ll * 4 * (dev.n_cu > 0 ? dev.n_cu : 256);
+    const long long tiles = (want + nch - 1) / nch;
+    if (tiles > (ns + ts - 1) / ts) {
+      const long long t = (ns + tiles - 1) / tiles;
+      ts = (int)(t < 16 ? (ns < 16 ? ns : 16) : t);
+    }
+  }
   if (nch > 0 && ns > 0) {
     const long long max_tiles = ((1ll << 31) - 1) / nch;
     if (max_tiles < 1)
@@ -375,6 +385,14 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
     A.d_sc = make_fastdiv((uint32_t)n_sc);
     A.sm_items = ((total + sg - 1) / sg) * n_sc;
     grid = (A.sm_items + wpb - 1) / wpb;
+    if (!(L && L->blocks_per_cu)) {
+      // every block ends with a flush of its image (an fp64 atomic per lit
+      // pixel): by default a block gets at least 8 items per wave (measured
+      // best from 256 to 16384 C3 sources), from 4 to 64 blocks per CU
+      const long long per8 = (A.sm_items + 8ll * wpb - 1) / (8ll * wpb);
+      const long long lo = 4ll * dev.n_cu;
+      grid = per8 < lo ? (grid < lo ? grid : lo) : per8;
+    }
   } else if (mode == IACT_MODE_RESPONSE) {
     grid = ns;
   } else {

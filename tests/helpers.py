@@ -161,22 +161,20 @@ def _discriminant_ambiguous(o, d, obs, k=32.0):
 BAND_METRES = 2.5e-5
 
 
-def band_units(sensor, px_band=1e-3, metres=BAND_METRES):
-    """The edge band in the sensor's margin units (pixel pitches for square
-    sensors, inradii for hexagonal ones): px_band, or `metres` on the sensor
-    when that is wider (sensors with millimetre pixels)."""
+def band_units(sensor, metres=BAND_METRES):
+    """The edge band, `metres` on the sensor, in the sensor's margin units
+    (pixel pitches for square sensors, inradii for hexagonal ones)."""
     if isinstance(sensor, O.SquareSensorSpec):
         x0, y0, dx, dy = sensor.params()
         unit = min(dx, dy)
     else:
         unit = sensor.inradius
-    return max(px_band, metres / unit)
+    return metres / unit
 
 
-def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
+def band_mask(sc, sources, values, source_type, metres=BAND_METRES, ob_eps=2e-5):
     """Rays (facet-major debug order) whose fate an fp32 rounding can flip:
-    sensor coordinate within px_band pixels (or hex-norm units) of a binning
-    boundary; a shadow/aperture decision that changes when obstructions and
+    sensor coordinate within the edge band of a binning boundary; a shadow/aperture decision that changes when obstructions and
     stage>=1 apertures are grown or shrunk by ob_eps metres (float64 oracle);
     or one that the reference's fp32 arithmetic (the ref32 oracle, no fused
     multiply-adds) gets differently from float64; or a primary-leg shadow
@@ -184,17 +182,17 @@ def band_mask(sc, sources, values, source_type, px_band=1e-3, ob_eps=2e-5):
     bound (_discriminant_ambiguous): there any two fp32 evaluations -- with
     or without fused multiply-adds, the kernels' or XLA's -- may disagree.
 
-    The edge band is px_band = 1e-3 pixels (hex-norm units for hexagonal
-    sensors) or 2.5e-5 m on the sensor, whichever is wider (band_units).  The
-    largest distance between a kernel's fp32 sensor coordinate and the
-    float64 one over every per-ray parity case is 8.6e-6 m (the C3 dish,
-    focal length 36 m: 4.1e-4 of its 2.1 cm inradius), so the band is 2.9
-    times that; the rays the kernels actually bin differently lie within
-    9.1e-4 px of an edge on CT3's 1 mm-pixel lid camera and within 1.2e-4 px
-    on the others (each parity case prints both figures).
+    The edge band is 2.5e-5 m on the sensor (band_units: 5e-4 of C2's 5 cm
+    pixels, 1.2e-3 of C3's 2.1 cm hexagon inradius, 0.025 of CT3's 1 mm lid
+    pixels).  The largest distance between a kernel's fp32 sensor coordinate
+    and the float64 one over every per-ray parity case is 8.6e-6 m (the C3
+    dish, focal length 36 m), so the band is 2.9 times that; the rays the
+    kernels actually bin differently lie within 9.1e-4 px (9.1e-7 m) of an
+    edge on CT3's lid camera and within 1.2e-4 px on the others (each parity
+    case prints both figures).
     """
     x, y, v, _ = O.render(sc, sources, values, source_type, "fp64", "debug")
-    band = O.edge_margin(x, y, sc.sensor) < band_units(sc.sensor, px_band)
+    band = O.edge_margin(x, y, sc.sensor) < band_units(sc.sensor, metres)
     alive = v != 0
     if sc.obstructions.cylinders or sc.obstructions.spheres:
         wp = O.transform_to_world(sc, "fp64")[0]                # (nf, M, 3)

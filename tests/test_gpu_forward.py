@@ -3,7 +3,7 @@
 Each case traces the same injected MC samples through the kernels and the
 oracle (reference fp32 arithmetic with fp64 source directions and Newton):
   * per-ray pixel ids must be identical except for rays in the edge band
-    (within 1e-3 px of a binning boundary; a shadow/aperture decision that
+    (within 2.5e-5 m of a binning boundary; a shadow/aperture decision that
     flips when obstructions and apertures move by 2e-5 m in the fp64 oracle;
     or a cylinder/sphere discriminant within its fp32 rounding bound --
     tests.helpers.band_mask);

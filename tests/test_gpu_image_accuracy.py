@@ -15,7 +15,7 @@ same samples (every step in double, image summed in double):
   * on every significant pixel (>= 1e-3 of the peak), the error that the edge
     band does not explain <= 1e-5 of the pixel: |gpu - fp64| minus the band
     mass of that pixel (|value| of the FP64 rays within the edge band of a
-    binning boundary -- 1e-3 px or 2.5e-5 m, tests.helpers.band_units -- at
+    binning boundary -- 2.5e-5 m on the sensor, tests.helpers.band_units -- at
     their pixel and the one across the nearest edge; the kernels' sensor
     coordinates stay within 8.6e-6 m of the FP64 ones, see the per-ray tests'
     printed coordinate errors);
