@@ -11,11 +11,13 @@ the whole iteration (a ray = one (source, facet, sample) triple, traced
 forward and backward).  Inputs are resident in HBM before timing starts.
 
 Also reported:
-  * roofline of the dominant kernel, the batched adjoint k_trace_adj: its
-    algorithmic FP32 flops per launch (rays x 2 x the forward's FP32 flops
-    per ray, the backward's share under the fwd+bwd = 3 x fwd convention,
-    no credit for obstruction tests) over its HIP-event time, against the
-    FP32 vector peak; the whole iteration's fraction beside it;
+  * roofline of the dominant kernel, the source-major adjoint k_adj_sm
+    (per instance): its algorithmic FP32 flops per batched backward (rays x
+    2 x the forward's FP32 flops per ray, the backward's share under the
+    fwd+bwd = 3 x fwd convention, no credit for obstruction tests) over its
+    HIP-event time, against the FP32 vector peak, with the VALU issue rate
+    and HBM bytes of the committed rocprofv3 counter summary; the whole
+    iteration's fraction beside it;
   * the same iteration done plainly (per-instance render + autograd);
   * image parity of batched instances against oracle/cpu_ref's FP64 run
     (and, with the C2 extra, of the full 6.4e9-ray C2 image);
@@ -176,20 +178,33 @@ def c5(ia, configs, n_instances, steps, warmup, want_parity, want_plain):
         "roofline": {
             "bound": "valu", "achieved": bwd_tf, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": bwd_tf * 1e12 / PEAK_FP32,
-            "traffic": profile_value("k_trace_adj_C5_counters.txt", "hbm_bytes_per_launch"),
-            "kernel": "k_trace_adj (batched adjoint, all instances in one launch), HIP events "
-                      "around iact_trace_batched_bwd on the launch stream, mean of 3",
+            "traffic": profile_value("k_adj_sm_C5_counters.txt", "hbm_bytes_per_launch"),
+            "kernel": "k_adj_sm (source-major adjoint; PART 1 and PART 2 launches per "
+                      "instance), HIP events around iact_trace_batched_bwd on the launch "
+                      "stream, mean of 3",
             "kernel_ms": b_ms,
+            "rocprof_kernel_ms": profile_value("k_adj_sm_C5_counters.txt",
+                                               "kernel_ms_per_backward"),
             "fp32_flops_per_ray": {"forward": f32, "adjoint": 2 * f32,
-                                   "convention": "fwd+bwd = 3 x fwd; no credit for obstruction "
-                                                 "tests; the 12 FP64 flops of a point source's "
-                                                 "direction (formed once per facet and source) "
-                                                 "not counted"},
-            "valu_issue": profile_value("k_trace_adj_C5_counters.txt", "valu_issue_frac"),
+                                   "convention": "fwd+bwd = 3 x fwd; FP32 only; no credit for "
+                                                 "obstruction tests; the 12 FP64 flops of a "
+                                                 "point source's direction not counted"},
+            "peak_note": "157.3 TFLOP/s assumes packed FP32 (two FMAs per lane per "
+                         "instruction); these kernels are built without packed FP32.  frac "
+                         "counts algorithmic flops, which include work the kernels do once "
+                         "per facet and source instead of per ray, and the adjoint at twice "
+                         "the forward's flops; valu_issue (rocprofv3 counters: VALU "
+                         "instructions x 4 cycles / (SIMDs x cycles)) is the hardware's "
+                         "measured utilisation",
+            "valu_issue": profile_value("k_adj_sm_C5_counters.txt", "valu_issue_frac"),
         },
         "forward_kernel": {"kernel": "batched forward (k_trace_sm per instance)",
                            "ms": f_ms, "achieved_tflops": fwd_tf,
-                           "frac": fwd_tf * 1e12 / PEAK_FP32},
+                           "frac": fwd_tf * 1e12 / PEAK_FP32,
+                           "rocprof_kernel_ms": profile_value("k_trace_sm_C5_counters.txt",
+                                                              "kernel_ms_per_forward"),
+                           "valu_issue": profile_value("k_trace_sm_C5_counters.txt",
+                                                       "valu_issue_frac")},
         "iteration_fp32_frac": it_tf * 1e12 / PEAK_FP32,
     }
     if want_plain:

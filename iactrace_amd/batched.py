@@ -5,8 +5,12 @@ leaves fitting to the user (jax.grad + optax over the parameters picked by
 utils/filtering.py:14-112).  Here many instances of one telescope -- same
 facet samples, obstructions and sensor, each with its own stage-0 rotations
 (e.g. independent tilt draws) and its own, unevenly sized source list -- are
-traced in ONE persistent launch (``iact_trace_batched_fwd``); the adjoint
-gives per-instance rotation gradients (``iact_trace_batched_bwd``).
+traced by ``iact_trace_batched_fwd`` and differentiated by
+``iact_trace_batched_bwd`` (per-instance rotation gradients).  For
+primary-only scenes these run the plain source-major kernels instance by
+instance on one stream (each instance already fills the GPU); the
+sample-major layout and scenes with stage >= 1 mirrors use one persistent
+launch over every (instance, facet chunk, source tile) work item.
 
 ``AlignmentFit`` runs the loop  image -> L2 loss vs a target -> rotation
 gradient -> Adam  entirely on the device and, by default, replays it as one

@@ -378,12 +378,24 @@ def render(tel, sources, values, source_type="point", sensor_idx=0, launch=None)
 
     Returns the accumulated image with the sensor's shape ((H, W) square,
     (P,) hexagonal).  Differentiable w.r.t. sources, values, stage-0 facet
-    rotations/positions and every surface's aspheric coefficients.
+    rotations/positions and every surface's aspheric coefficients; raises
+    NotImplementedError when a stage >= 1 mirror's position or rotation
+    requires grad (the reference differentiates those through
+    _reflect_at_stage, render.py:44-79).
     """
     plan = _plan(tel, sensor_idx, launch)
     shape = plan.sensor.get_accumulator_shape()
     sources = as_tensor(sources)
     values = as_tensor(values)
+    if torch.is_grad_enabled():
+        for stage, groups in plan.secondary:
+            for g in groups:
+                if g.positions.requires_grad or g.rotations.requires_grad:
+                    raise NotImplementedError(
+                        f"gradients with respect to the positions or rotations of stage-{stage} "
+                        "mirrors are not implemented (the adjoint covers stage-0 poses, "
+                        "sources, values and aspheric coefficients); detach them or render "
+                        "under torch.no_grad()")
     if plan.empty:
         return torch.zeros(shape, device=plan.device)
     rot = torch.cat([g.rotations for g in plan.primary], 0) if len(plan.primary) > 1 \

@@ -289,9 +289,15 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
                        const iact_grads* grads, void* stream);
 
 /* ---- batched (no reference counterpart: the reference renders one
- * telescope per call).  Persistent launch over (instance, facet chunk,
- * source tile) work items; images (n_instances, n_pixels) and, for the
- * backward, per-instance rotation gradients (n_instances, n_facets, 3).     */
+ * telescope per call).  Many instances of one telescope, each with its own
+ * stage-0 rotations and source list; images (n_instances, n_pixels) and, for
+ * the backward, per-instance rotation gradients (n_instances, n_facets, 3).
+ * Primary-only scenes (the default launch) run the plain source-major
+ * forward and adjoint instance by instance on one stream, sharing the
+ * workspace; with layout 1 or deterministic launches, and for scenes with
+ * stage >= 1 mirrors, one persistent sample-major launch covers every
+ * (instance, facet chunk, source tile) work item.  The batched backward
+ * covers primary-only scenes.                                               */
 size_t iact_batch_workspace_size(const iact_scene* scene, const iact_batch* batch,
                                  const iact_launch* launch);
 int32_t iact_trace_batched_fwd(const iact_scene* scene, const iact_batch* batch,

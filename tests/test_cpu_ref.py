@@ -1,4 +1,4 @@
-"""The C/OpenMP restatement (CPU baseline) agrees with the numpy oracle."""
+"""The C++/OpenMP restatement (CPU baseline, FP64 run, adjoint) agrees with the numpy oracle and the twin."""
 import numpy as np
 import pytest
 

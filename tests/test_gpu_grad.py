@@ -418,3 +418,22 @@ def test_source_major_adjoint(case):
     _compare(f"source-major {case}", sm, gt, keys, tol=1e-3)
     _compare(f"source-major {case} (agreeing facets)", sm, gt,
              [k for k in keys if k != "src"], tol=1e-4, facet_mask=keep)
+
+
+def test_stage_mirror_pose_gradients_refused():
+    """A stage >= 1 mirror's pose that requires grad is refused with a clear
+    error (the adjoint covers stage-0 poses, sources, values and aspheric
+    coefficients), not silently left without a gradient."""
+    wl = configs.workload("C4", n_samples=64)
+    tel = wl.telescope(DEV, key=1)
+    g0, g1 = tel.mirror_groups
+    src = torch.as_tensor(wl.sources[:16])
+    vals = torch.as_tensor(wl.values[:16])
+    for field in ("rotations", "positions"):
+        leaf = getattr(g1, field).detach().clone().requires_grad_(True)
+        t2 = ia.Telescope([g0, g1.replace(**{field: leaf})], [], tel.sensors)
+        with pytest.raises(NotImplementedError, match="stage-1"):
+            ia.render(t2, src, vals, wl.source_type)
+        with torch.no_grad():
+            img = ia.render(t2, src, vals, wl.source_type)
+        assert float(img.sum()) > 0
