@@ -265,7 +265,28 @@ IACT_DEV double newton_t(const SurfT<double>& s, double ox, double oy, double oz
 // ---------------------------------------------------------------- obstructions
 // Each returns the ray parameter of the nearest hit or +inf
 // (intersections.py:44-226).  The shadow test only asks t < 1e10.
-IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
+// The cylinder test in two parts: the terms of the ray direction alone
+// (cyl_dir), which a run of rays with one direction forms once, and the
+// rest (hit_cylinder_dir).  hit_cylinder is their composition: every caller
+// evaluates the same expressions.
+struct CylDir {
+  f3 w, rdp;
+  float ww, rda, a, ia, ir;
+};
+IACT_DEV CylDir cyl_dir(f3 d, cptr<float> c) {
+  const float eps = 1e-8f;
+  const f3 ax = mk3(c[3], c[4], c[5]);
+  CylDir D;
+  D.w = cross(d, ax);
+  D.ww = dot(D.w, D.w);
+  D.rda = dot(d, ax);
+  D.rdp = d - D.rda * ax;
+  D.a = dot(D.rdp, D.rdp);
+  D.ia = frcp(2.f * D.a + eps);
+  D.ir = frcp(D.rda + eps);
+  return D;
+}
+IACT_DEV float hit_cylinder_dir(f3 o, const CylDir& D, cptr<float> c) {
   const float eps = 1e-8f;
   const f3 p1 = mk3(c[0], c[1], c[2]);
   const f3 ax = mk3(c[3], c[4], c[5]);
@@ -280,41 +301,48 @@ IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
   // NaN geometry never take it.  The per-lane result is the miss for such a
   // lane whatever its wave, and the rest of the test is skipped when it
   // holds for the whole wave.
-  const f3 w = cross(d, ax);
-  const float ow = dot(oc, w);
-  const bool far = ow * ow > dot(w, w) * fmaf(1e-5f, dot(oc, oc), 1.0002f * r * r);
+  const float ow = dot(oc, D.w);
+  const bool far = ow * ow > D.ww * fmaf(1e-5f, dot(oc, oc), 1.0002f * r * r);
   if (__all(far)) return inf;
-  const float oca = dot(oc, ax), rda = dot(d, ax);
-  const f3 ocp = oc - oca * ax, rdp = d - rda * ax;
-  const float a = dot(rdp, rdp);
-  const float b = 2.f * dot(ocp, rdp);
+  const float oca = dot(oc, ax);
+  const f3 ocp = oc - oca * ax;
+  const float b = 2.f * dot(ocp, D.rdp);
   const float cc = dot(ocp, ocp) - r * r;
-  const float disc = b * b - 4.f * a * cc;
+  const float disc = b * b - 4.f * D.a * cc;
   const float sq = fsqrt(fmaxf(disc, 0.f));
-  const float ia = frcp(2.f * a + eps);
-  float t1 = (-b - sq) * ia, t2 = (-b + sq) * ia;
-  const float y1 = oca + t1 * rda, y2 = oca + t2 * rda;
+  float t1 = (-b - sq) * D.ia, t2 = (-b + sq) * D.ia;
+  const float y1 = oca + t1 * D.rda, y2 = oca + t2 * D.rda;
   t1 = (t1 > eps && y1 >= 0.f && y1 <= h && disc >= 0.f) ? t1 : inf;
   t2 = (t2 > eps && y2 >= 0.f && y2 <= h && disc >= 0.f) ? t2 : inf;
-  const float ir = frcp(rda + eps);
-  float tb = -oca * ir, tt = (h - oca) * ir;
-  const f3 pb = ocp + tb * rdp, pt = ocp + tt * rdp;
+  float tb = -oca * D.ir, tt = (h - oca) * D.ir;
+  const f3 pb = ocp + tb * D.rdp, pt = ocp + tt * D.rdp;
   tb = (tb > eps && dot(pb, pb) <= r * r) ? tb : inf;
   tt = (tt > eps && dot(pt, pt) <= r * r) ? tt : inf;
   return far ? inf : fminf(fminf(t1, t2), fminf(tb, tt));
 }
+IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
+  return hit_cylinder_dir(o, cyl_dir(d, c), c);
+}
 
-IACT_DEV float hit_box(f3 o, f3 d, cptr<float> b) {
+// The axis-aligned box test in two parts, as the cylinder's.
+struct BoxDir {
+  float ix, iy, iz;
+};
+IACT_DEV BoxDir box_dir(f3 d) {
   const float eps = 1e-8f;
-  const float ix = frcp(d.x + eps), iy = frcp(d.y + eps), iz = frcp(d.z + eps);
-  const float tx1 = (b[0] - o.x) * ix, tx2 = (b[3] - o.x) * ix;
-  const float ty1 = (b[1] - o.y) * iy, ty2 = (b[4] - o.y) * iy;
-  const float tz1 = (b[2] - o.z) * iz, tz2 = (b[5] - o.z) * iz;
+  return BoxDir{frcp(d.x + eps), frcp(d.y + eps), frcp(d.z + eps)};
+}
+IACT_DEV float hit_box_dir(f3 o, BoxDir D, cptr<float> b) {
+  const float eps = 1e-8f;
+  const float tx1 = (b[0] - o.x) * D.ix, tx2 = (b[3] - o.x) * D.ix;
+  const float ty1 = (b[1] - o.y) * D.iy, ty2 = (b[4] - o.y) * D.iy;
+  const float tz1 = (b[2] - o.z) * D.iz, tz2 = (b[5] - o.z) * D.iz;
   const float tmin = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
   const float tmax = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2));
   const bool hit = (tmax >= tmin) && (tmax > eps);
   return hit ? (tmin > eps ? tmin : tmax) : __builtin_inff();
 }
+IACT_DEV float hit_box(f3 o, f3 d, cptr<float> b) { return hit_box_dir(o, box_dir(d), b); }
 
 IACT_DEV float hit_sphere(f3 o, f3 d, cptr<float> s) {
   const float eps = 1e-8f;

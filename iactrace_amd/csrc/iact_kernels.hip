@@ -805,10 +805,12 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     bool refined = false;
     int rlist[SPL];
     uint64_t rbits[SPL];
+    uint64_t blk[SPL];  // shadow bits of a far run (far_run_shadow)
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
       run_pix[k] = -1;
       run_v[k] = 0.f;
+      blk[k] = 0ull;
     }
     // one sample's rays, with shadow tests (SH) or without; the shadow
     // kernel forms the reflection from n (its records' m words stay unread:
@@ -816,10 +818,11 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     // no parallel-ray test, no edge test and square pixels (the launch needs
     // neither test and has such pixels).  FR: every lane's sources are far
     // (source_qs 0), u = e.
-    auto trace_one = [&](auto sh, auto gen, auto far, const RecRaw& rec) {
+    auto trace_one = [&](auto sh, auto gen, auto far, auto sbits, const RecRaw& rec, int bi) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
       constexpr bool GEN = decltype(gen)::value;
       constexpr bool FR = decltype(far)::value;
+      constexpr bool SB = decltype(sbits)::value;
       const Sample smp0 = unpack_record(rec);
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
@@ -833,8 +836,16 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
-        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0, GEN, FR>(
-            A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
+        // (the shadow kernel forms the reflection from n in every loop)
+        const bool ok =
+            trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0 && PART != 2, GEN, FR>(
+                A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
+        if (SB) {
+          // the run's shadow bits (far_run_shadow) in place of the test
+          const float sb = ((blk[k] >> bi) & 1ull) ? 0.f : 1.f;
+          v *= sb;
+          vf *= sb;
+        }
         if (!SPLAT) {
           // run of equal pixels per lane and source (consecutive samples of
           // a facet put a source's rays on the same one or two pixels), added
@@ -897,12 +908,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       // the next record is loaded while one is traced (the record table has
       // padding at its end, so the prefetch needs no clamping)
       const float4* rp = recs + 4 * (size_t)j;
+      const int jb = j;
       RecRaw cur = fetch_record(rp);
-      auto run = [&](auto sh, auto gen, auto far) {
+      auto run = [&](auto sh, auto gen, auto far, auto sbits) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
-          trace_one(sh, gen, far, cur);
+          trace_one(sh, gen, far, sbits, cur, j - jb);
           cur = nxt;
         }
       };
@@ -911,17 +923,29 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
       if (OBST && PART == 2) {
         // (no generic copy: the shadow tests dominate, and more copies would
         // cost this kernel scalar registers)
-        if (all_far)
-          run(T_{}, T_{}, T_{});
-        else
-          run(T_{}, T_{}, F_{});
+        if (all_far && refined) {
+          // far sources over a refined run: the candidates' shadow bits
+          // first, then the run without shadow tests
+#pragma unroll
+          for (int k = 0; k < SPL; ++k) {
+            const f3 u = e[k];
+            const f3 d = frsq(dot(u, u)) * u;  // trace_ray's d of a far ray
+            blk[k] = far_run_shadow(A, (const float*)(recs + 4 * (size_t)jb), jend - jb,
+                                    rlist[k], rbits[k], -d);
+          }
+          run(F_{}, T_{}, T_{}, T_{});
+        } else if (all_far) {
+          run(T_{}, T_{}, T_{}, F_{});
+        } else {
+          run(T_{}, T_{}, F_{}, F_{});
+        }
       } else if (lean) {
         if (all_far)
-          run(F_{}, F_{}, T_{});
+          run(F_{}, F_{}, T_{}, F_{});
         else
-          run(F_{}, F_{}, F_{});
+          run(F_{}, F_{}, F_{}, F_{});
       } else {
-        run(F_{}, T_{}, F_{});
+        run(F_{}, T_{}, F_{}, F_{});
       }
     }
     if (!SPLAT) {
@@ -1017,15 +1041,18 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
     // (also per ray, in the reference's fp32 gradient), and are left out.
     f3 fd[SPL], fN[SPL];
     float finv[SPL], fC[SPL];
+    uint64_t blk[SPL];  // shadow bits of a far run (far_run_shadow)
     const bool want_pc = want_pos || want_src;
     // one sample's rays (SPL per lane): the forward's trace up to the pixel,
     // then the lane's share of the sample's torque and dL/dp (tq, gp), or
     // (FR) the far sums
-    auto adjoint_one = [&](auto sh, auto gen, auto far, const RecRaw& rec, f3& tq, f3& gp) {
+    auto adjoint_one = [&](auto sh, auto gen, auto far, auto sbits, const RecRaw& rec, int bi,
+                           f3& tq, f3& gp) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
       constexpr bool GEN = decltype(gen)::value;
       constexpr bool FR = decltype(far)::value;
-      constexpr bool RM = OB == 0;
+      constexpr bool SB = decltype(sbits)::value;
+      constexpr bool RM = OB == 0 && PART != 2;
       const Sample smp0 = unpack_record(rec);
       float san_x = 0.f, san_y = 0.f, san_z = 0.f, st = 0.f;
       float sw_x = 0.f, sw_y = 0.f, sw_z = 0.f;
@@ -1061,16 +1088,17 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
           rs = 1.f;
           u = d;
         }
-        const float shadow = primary_shadow<OB>(A, smp, d, true);
+        const float shadow = SB ? (((blk[k] >> bi) & 1ull) ? 0.f : 1.f)
+                                : primary_shadow<OB>(A, smp, d, true);
         float gu, gv;
         const bool ok = plane_grid<KIND, AXIS, GEN>(A.S, smp.G, r, gu, gv, rs);
         const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
         const float gp_pix = pix >= 0 ? G[pix] : 0.f;
         // a = G value shadow; dc = a nrw is applied per sample
-        const float a = OB ? gp_pix * value[k] * shadow : gp_pix * value[k];
+        const float a = (OB || SB) ? gp_pix * value[k] * shadow : gp_pix * value[k];
         if (want_val) {
           // dL/dvalue = G v / value = G shadow (d.m) nrwh
-          const float gs = OB ? gp_pix * shadow : gp_pix;
+          const float gs = (OB || SB) ? gp_pix * shadow : gp_pix;
           gval[k] = fmaf(gs * (c2u * inv), smp.nrwh, gval[k]);
         }
         if (SRC == IACT_SOURCE_POINT && FR) {
@@ -1155,12 +1183,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
       const float4* rp = recs + 4 * (size_t)j;
       RecRaw cur = fetch_record(rp);
       f3 T = mk3(0.f, 0.f, 0.f), Gp = mk3(0.f, 0.f, 0.f);
-      auto run = [&](auto sh, auto gen, auto far) {
+      const int jb = j;
+      auto run = [&](auto sh, auto gen, auto far, auto sbits) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
           f3 tq, gp;
-          adjoint_one(sh, gen, far, cur, tq, gp);
+          adjoint_one(sh, gen, far, sbits, cur, j - jb, tq, gp);
           if (!decltype(far)::value) {
             T = T + tq;
             Gp = Gp + gp;
@@ -1171,17 +1200,26 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
       using T_ = std::true_type;
       using F_ = std::false_type;
       if (OBST && PART == 2) {
-        if (all_far)
-          run(T_{}, T_{}, T_{});
-        else
-          run(T_{}, T_{}, F_{});
+        if (all_far && refined) {
+          // the forward's shadow bits of the run (far_run_shadow), then the
+          // run without shadow tests
+#pragma unroll
+          for (int k = 0; k < SPL; ++k)
+            blk[k] = far_run_shadow(A, (const float*)(recs + 4 * (size_t)jb), jend - jb,
+                                    rlist[k], rbits[k], -fd[k]);
+          run(F_{}, T_{}, T_{}, T_{});
+        } else if (all_far) {
+          run(T_{}, T_{}, T_{}, F_{});
+        } else {
+          run(T_{}, T_{}, F_{}, F_{});
+        }
       } else if (lean) {
         if (all_far)
-          run(F_{}, F_{}, T_{});
+          run(F_{}, F_{}, T_{}, F_{});
         else
-          run(F_{}, F_{}, F_{});
+          run(F_{}, F_{}, F_{}, F_{});
       } else {
-        run(F_{}, T_{}, F_{});
+        run(F_{}, T_{}, F_{}, F_{});
       }
       if (all_far) {
 #pragma unroll

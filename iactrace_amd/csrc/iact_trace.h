@@ -553,6 +553,52 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
                               s.cand_any, s.p, -d, active);
 }
 
+// Shadow bits of a facet run of far sources (source_qs 0): bit i is set when
+// the shadow ray of the run's sample i towards the lane's source -- the
+// direction dsh = -d, fixed over the run -- meets one of the candidates that
+// shadow_factor_list would test (list, bits: one obstruction per lane, the
+// wave-uniform selection).  Candidates are the outer loop, so a cylinder's
+// and a box's direction terms are formed once per run; the tests are the
+// per-ray kernels' own expressions (hit_cylinder = hit_cylinder_dir of
+// cyl_dir, and so on), so every ray is decided as they decide it.  The run's
+// n <= 64 sample points come one per lane (vector loads of the records' p
+// words) and are handed out with v_readlane.
+IACT_DEV uint64_t far_run_shadow(const TraceArgs& A, const float* rec0, int n, int list,
+                                 uint64_t bits, f3 dsh) {
+  const iact_obstructions& ob = A.ob;
+  const int lane = lane_id();
+  const float* pr = rec0 + (size_t)IACT_REC_WORDS * (lane < n ? lane : 0) + 13;
+  const f3 myp = mk3(pr[0], pr[1], pr[2]);
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes;
+  uint64_t blocked = 0ull;
+  while (bits != 0ull) {
+    const int pos = __builtin_ctzll(bits);
+    bits &= bits - 1ull;
+    const int j = __builtin_amdgcn_readlane(list, pos);
+    if (j < c0) {
+      const cptr<float> c = as_const(ob.cylinders) + 8 * j;
+      const CylDir D = cyl_dir(dsh, c);
+      for (int i = 0; i < n; ++i) {
+        if (__all((blocked >> i) & 1ull)) continue;
+        if (hit_cylinder_dir(lane_value(myp, i), D, c) < 1e10f) blocked |= 1ull << i;
+      }
+    } else if (j < c1) {
+      const cptr<float> b = as_const(ob.boxes) + 6 * (j - c0);
+      const BoxDir D = box_dir(dsh);
+      for (int i = 0; i < n; ++i) {
+        if (__all((blocked >> i) & 1ull)) continue;
+        if (hit_box_dir(lane_value(myp, i), D, b) < 1e10f) blocked |= 1ull << i;
+      }
+    } else {
+      for (int i = 0; i < n; ++i) {
+        if (__all((blocked >> i) & 1ull)) continue;
+        if (hit_obstruction(ob, j, lane_value(myp, i), dsh) < 1e10f) blocked |= 1ull << i;
+      }
+    }
+  }
+  return blocked;
+}
+
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
 // GEN false (image traces of launches whose samples all satisfy
 // plane_test_redundant, on square pixels) leaves out the parallel-ray test
