@@ -752,11 +752,28 @@ IACT_DEV int trunc_i32(float x) {
 // and the edge test only for sensors with an edge.  Without one
 // (edge_thr = 1) the reference's test can only drop a point whose hex norm
 // rounds above 1, i.e. a point within rounding of a cell boundary.
+// Nearest hex centre from three floors: with cube coordinates x = q,
+// z = r, y = -q - r, every cell boundary lies on an integer line of
+// a = x - y = 2q + r, b = y - z = -q - 2r or c = z - x = r - q, and on each
+// triangle those lines cut out the centre is ((A - C + 1) div 3,
+// (C - B + 1) div 3) for A, B, C the floors (checked against cube rounding on
+// the CPU).  k div 3 is floor(k/3 + 1/2) for an integer k: 1/6 clear of
+// every integer, so the fp32 product is safe.  The cell of hex_round except
+// for points within rounding of a cell boundary; 12 vector instructions
+// with the conversions to int.
+IACT_DEV void hex_cell_floor(float q, float r, int& iq, int& ir) {
+  const float A = floorf(fmaf(2.f, q, r));
+  const float B = floorf(fmaf(-2.f, r, -q));
+  const float C = floorf(r - q);
+  iq = floor_i32(fmaf(A - C, 1.f / 3.f, 0.5f));
+  ir = floor_i32(fmaf(C - B, 1.f / 3.f, 0.5f));
+}
+
 template <bool EDGE = true>
 IACT_DEV int hex_pixel_hit(const DevSensor& S, const int32_t* table, float q, float r, bool ok) {
-  float qi, ri;
-  hex_round_nearest(q, r, qi, ri);
-  const int iq = trunc_i32(qi), ir = trunc_i32(ri);
+  int iq, ir;
+  hex_cell_floor(q, r, iq, ir);
+  const float qi = (float)iq, ri = (float)ir;
   const bool in = ((unsigned)iq < (unsigned)S.tq) & ((unsigned)ir < (unsigned)S.tr);
   const unsigned last = __umul24((unsigned)S.tq, (unsigned)S.tr) - 1u;
   const int pid = table[min(__umul24((unsigned)iq, (unsigned)S.tr) + (unsigned)ir, last)];
