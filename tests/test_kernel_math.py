@@ -53,3 +53,45 @@ def test_cylinder_early_out_never_skips_a_hit():
     assert 0.2 < far.mean() < 0.95 and hit32.mean() > 0.02
     assert not np.any(far & hit32)
     assert not np.any(far & hit64g)
+
+
+def _hex_cell_floor(q, r):
+    """hex_cell_floor (iact_device.h) in fp32: the centre from the floors of
+    the three cube-coordinate differences."""
+    f = np.float32
+    q, r = q.astype(f), r.astype(f)
+    A = np.floor(f(2) * q + r)
+    B = np.floor(-q - f(2) * r)
+    C = np.floor(r - q)
+    third = f(1.0 / 3.0)
+    return (np.floor((A - C) * third + f(0.5)).astype(np.int64),
+            np.floor((C - B) * third + f(0.5)).astype(np.int64))
+
+
+def test_hex_cell_floor_is_cube_rounding():
+    """The kernels' nearest-hexagon formula against cube rounding (the
+    oracle's axial_round, hexagonal.py:32-39) on random axial points: the
+    same cell for every point farther than 1e-4 from a cell boundary (every
+    boundary lies on an integer line of 2q + r, -q - 2r or r - q), and a
+    planted off-by-one in the division is caught."""
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-300.0, 300.0, 400_000)
+    r = rng.uniform(-300.0, 300.0, 400_000)
+    qi, ri = O.axial_round(q, r)
+    # distance (in the pairwise differences) from the nearest integer line
+    d = np.stack([2 * q + r, -q - 2 * r, r - q])
+    margin = np.abs(d - np.round(d)).min(0)
+    clear = margin > 1e-4
+    fq, fr = _hex_cell_floor(q, r)
+    assert clear.mean() > 0.99
+    assert np.array_equal(fq[clear], np.asarray(qi)[clear])
+    assert np.array_equal(fr[clear], np.asarray(ri)[clear])
+    # integer centres map to themselves, also at the table's far corner
+    c = np.arange(-5, 300, dtype=np.float64)
+    cq, cr = _hex_cell_floor(c, c[::-1].copy())
+    assert np.array_equal(cq, c.astype(np.int64)) and np.array_equal(cr, c[::-1].astype(np.int64))
+    # planted mistake: (A - C) div 3 without the +1 misses the cells
+    A = np.floor(2 * q + r)
+    C = np.floor(r - q)
+    bad = np.floor((A - C) / 3.0).astype(np.int64)
+    assert not np.array_equal(bad[clear], np.asarray(qi)[clear])
