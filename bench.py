@@ -43,6 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 METRIC = "rays/s fwd and fwd+bwd, batched vs plain, 1 MI355X at ≤1e-5 image rel err"
 
@@ -289,8 +290,16 @@ def cpu_fwd_bwd(bw, tel, rot, targets, reps=5):
     tf = statistics.median(r[0] for r in runs)
     tb = statistics.median(r[1] for r in runs)
     rays = nf * M * ns
+    host = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            host = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                        host)
+    except OSError:
+        pass
     return {"value": rays / (tf + tb), "unit": "rays/s (forward + backward)", "cores": nt,
-            "kind": "port",
+            "kind": "port", "host_cpu": host, "nproc": os.cpu_count(),
+            "extrapolated_full_iteration_s": bw.n_rays / (rays / (tf + tb)),
             "sample": f"C5 instance 0 with its first {ns} of {int(bw.offsets[1])} sources x "
                       f"{nf} facets x {M} samples ({rays:.3g} rays): oracle/cpu_ref.cc forward "
                       f"image ({1e3 * tf:.0f} ms) + adjoint of the L2 loss wrt rotations and "
@@ -417,11 +426,20 @@ def c4_response(ia, configs, steps, warmup):
 
     wr, _ = time_steps(lambda: ia.render_response_matrix(tel, src, vals, wl.source_type), steps,
                        warmup)
-    wi, _ = time_steps(lambda: ia.render(tel, src, vals, wl.source_type), steps, warmup)
+    wi, evi = time_steps(lambda: ia.render(tel, src, vals, wl.source_type), steps, warmup)
     wb, _ = time_steps(fwd_bwd, steps, warmup)
+    from iactrace_amd.flop_model import split_per_ray
+    f32, f64 = split_per_ray("parallel", 0, 0, 0, 1, 0, "square", len(wl.sources))
+    ki = float(sum(evi) / len(evi)) * 1e-3
     return {"workload": "C4: Cassegrain, 1 primary x 2048 samples x 8192 parallel sources "
                         "(128 x 64 grid, 0.01 deg), square 64 x 64 sensor",
             "rays_per_step": wl.n_rays, "steps": steps,
+            "image_roofline": {"kernel_ms": ki * 1e3, "fp32_flops_per_ray": f32,
+                               "fp64_flops_per_ray": f64,
+                               "fp32_frac": wl.n_rays * f32 / ki / PEAK_FP32,
+                               "fp64_frac": wl.n_rays * f64 / ki / PEAK_FP64,
+                               "note": "the secondary's Newton solve is FP64 (280 flops per "
+                                       "ray), weighed against the FP64 vector peak"},
             "response_matrix_rays_per_s": wl.n_rays * steps / wr,
             "response_matrix_ms": 1e3 * wr / steps,
             "image_rays_per_s": wl.n_rays * steps / wi, "image_ms": 1e3 * wi / steps,

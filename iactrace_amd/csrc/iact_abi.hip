@@ -200,6 +200,18 @@ DevSensor make_dev_sensor(const iact_sensor& s) {
     S.ary = (float)ary;
     S.ar0 = (float)(-(arx * ox + ary * oy) - (double)s.r_min);
   }
+  S.An = S.nrm;
+  if (s.kind == IACT_SENSOR_SQUARE) {
+    S.Au = S.u1;
+    S.Av = f3{(float)((double)S.v_scale * R[1]), (float)((double)S.v_scale * R[4]),
+              (float)((double)S.v_scale * R[7])};
+  } else {
+    const double a = S.aqx, b = S.aqy, c = S.arx, d = S.ary;
+    S.Au = f3{(float)(a * R[0] + b * R[1]), (float)(a * R[3] + b * R[4]),
+              (float)(a * R[6] + b * R[7])};
+    S.Av = f3{(float)(c * R[0] + d * R[1]), (float)(c * R[3] + d * R[4]),
+              (float)(c * R[6] + d * R[7])};
+  }
   // bounds for plane_test_redundant (grid units): the smallest stretch of an
   // in-plane direction by the grid map, and the grid's extent
   if (s.kind == IACT_SENSOR_SQUARE) {

@@ -372,12 +372,14 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
       if (!SPLAT) {
         float gu_, gv_;
         bool ok;
-        if (SRC == IACT_SOURCE_POINT && (!STAGES || A.n_stages == 0)) {
-          // the hit as trace_ray forms it (the reflection of u = q + e, not
-          // normalised), so that the pixel is the forward's
-          const f3 u = fma3(qs, smp.q, e);
-          const f3 ru = fma3(-2.f * dot(u, smp.n), smp.n, u);
-          ok = plane_grid<KIND, AXIS>(A.S, Kg, ru, gu_, gv_, inv) && active;
+        if (!STAGES || A.n_stages == 0) {
+          // the hit as trace_ray forms it (grid_hit of r = (u.m) n + u with
+          // u = q + e not normalised for point sources, u = d for parallel
+          // ones), so that the pixel is the forward's
+          const f3 u = SRC == IACT_SOURCE_POINT ? fma3(qs, smp.q, e) : d;
+          ok = grid_hit<true>(Kg, smp.N, sensor_dirs(A.S, u), dot(u, smp.m), gu_, gv_,
+                              SRC == IACT_SOURCE_POINT ? inv : 1.f) &&
+               active;
         } else {
           ok = plane_grid<KIND, AXIS>(A.S, Kg, rf, gu_, gv_) && active;
         }

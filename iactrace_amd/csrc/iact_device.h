@@ -517,6 +517,10 @@ struct DevSensor {
   int kind, splat, n_pixels;
   int axis;        // frame is the identity (u1 = x, u2 = y, normal = z)
   f3 pos, u1, u2, nrm;
+  // a direction r enters the hit only through r.An (= r.nrm) and its grid
+  // components r.Au, r.Av (square: u1 and v_scale u2; hexagonal: the axial
+  // rows of the grid map over u1, u2)
+  f3 An, Au, Av;
   float ndotp;
   float nx0, ny0;  // -x0/dx, -y0/dy: pixel coordinate = fma(x, 1/dx, -x0/dx)
   float v_scale;   // (1/dy) / (1/dx): 1 for square pixels
@@ -616,6 +620,27 @@ IACT_DEV bool plane_grid(const DevSensor& S, const GridK& G, f3 r, float& gu, fl
     gu = fmaf(t, fmaf(ru, S.aqx, rv * S.aqy), G.ku);
     gv = fmaf(t, fmaf(ru, S.arx, rv * S.ary), G.kv);
   }
+  return (!GEN || !(fabsf(nd * rs) < 1e-10f)) && t > 0.f;
+}
+
+// The hit of a primary ray in grid coordinates from its reflection written
+// as r = c n + u (c = u.m, m = -2 n): r.A = c (n.A) + u.A for each of the
+// sensor's directions A, so with N = (n.An, n.Au, n.Av) per sample and
+// U = (u.An, u.Au, u.Av) -- per (facet, source) when u is -- the ray needs
+// three FMAs for its normal and grid components.  The same miss conditions
+// as plane_grid (rs: 1/|u|).
+IACT_DEV f3 sensor_dirs(const DevSensor& S, f3 u) {
+  return mk3(dot(u, S.An), dot(u, S.Au), dot(u, S.Av));
+}
+template <bool GEN = true>
+IACT_DEV bool grid_hit(const GridK& G, f3 N, f3 U, float c, float& gu, float& gv,
+                       float rs = 1.f) {
+  const float nd = fmaf(c, N.x, U.x);
+  const float ru = fmaf(c, N.y, U.y);
+  const float rv = fmaf(c, N.z, U.z);
+  const float t = G.kt * frcp(nd);
+  gu = fmaf(t, ru, G.ku);
+  gv = fmaf(t, rv, G.kv);
   return (!GEN || !(fabsf(nd * rs) < 1e-10f)) && t > 0.f;
 }
 

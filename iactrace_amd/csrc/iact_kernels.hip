@@ -836,10 +836,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         float gu, gv, v, x, y, vf;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
-        // (the shadow kernel forms the reflection from n in every loop)
-        const bool ok =
-            trace_ray<SRC, KIND, STAGES, AXIS, OB, false, OB == 0 && PART != 2, GEN, FR>(
-                A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
+        const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, GEN, FR>(
+            A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
         if (SB) {
           // the run's shadow bits (far_run_shadow) in place of the test
           const float sb = ((blk[k] >> bi) & 1ull) ? 0.f : 1.f;
@@ -1052,7 +1050,6 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
       constexpr bool GEN = decltype(gen)::value;
       constexpr bool FR = decltype(far)::value;
       constexpr bool SB = decltype(sbits)::value;
-      constexpr bool RM = OB == 0 && PART != 2;
       const Sample smp0 = unpack_record(rec);
       float san_x = 0.f, san_y = 0.f, san_z = 0.f, st = 0.f;
       float sw_x = 0.f, sw_y = 0.f, sw_z = 0.f;
@@ -1067,7 +1064,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
         }
         // the forward's arithmetic (trace_ray): the same pixel
         float inv, c2u, rs;
-        f3 u, d, r;
+        f3 u, d;
         if (SRC == IACT_SOURCE_POINT) {
           if (FR) {
             u = e[k];
@@ -1078,20 +1075,17 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
             inv = frsq(dot(u, u));
             d = inv * u;
           }
-          c2u = RM ? dot(u, smp.m) : -2.f * dot(u, smp.n);
-          r = fma3(c2u, smp.n, u);
           rs = inv;
         } else {
           d = primary_direction<SRC>(smp, e[k], 1.f, inv);
-          c2u = RM ? dot(d, smp.m) : -2.f * dot(d, smp.n);
-          r = fma3(c2u, smp.n, d);
           rs = 1.f;
           u = d;
         }
+        c2u = dot(u, smp.m);
         const float shadow = SB ? (((blk[k] >> bi) & 1ull) ? 0.f : 1.f)
                                 : primary_shadow<OB>(A, smp, d, true);
         float gu, gv;
-        const bool ok = plane_grid<KIND, AXIS, GEN>(A.S, smp.G, r, gu, gv, rs);
+        const bool ok = grid_hit<GEN>(smp.G, smp.N, sensor_dirs(A.S, u), c2u, gu, gv, rs);
         const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
         const float gp_pix = pix >= 0 ? G[pix] : 0.f;
         // a = G value shadow; dc = a nrw is applied per sample

@@ -126,9 +126,9 @@ struct TraceArgs {
 };
 
 // World-space sample record of the source-major trace (16 words, read with
-// scalar loads): q = p - facet position, n, 1/(2 weight), the sensor-grid
-// constants of p, m = -2 n, p -- the fields of the Sample that make_sample
-// builds which the image trace uses.
+// scalar loads): q = p - facet position, N = (n.An, n.Au, n.Av), 1/(2 weight),
+// the sensor-grid constants of p, m = -2 n, p -- the fields of the Sample
+// that make_sample builds which the image trace uses (n = -m/2 exactly).
 #define IACT_REC_WORDS 16
 
 // Position of a block in an XCD-major order.  Workgroups are dispatched to
@@ -243,6 +243,7 @@ IACT_DEV bool decode_item(const TraceArgs& A, long long item, int inst, int lane
 struct Sample {
   f3 p, n;
   f3 m;            // -2 n: the reflection's scaled cosine is one dot product
+  f3 N;            // (n.An, n.Au, n.Av): the normal's part of the hit (grid_hit)
   f3 q;            // p - facet position, in fp32
   float nrw;       // -1 / weight (the reflected value is value * cos / w, cos = -d.n)
   float nrwh;      // -nrw / 2 (value * (d.m) * nrwh is value * (d.n) * nrw exactly)
@@ -306,6 +307,7 @@ IACT_DEV Sample make_sample(const DevSensor& S, f3 p, f3 n, float w, f3 facet_po
   s.nrwh = -0.5f * s.nrw;
   s.K = plane_consts(S, p);
   s.G = grid_consts<KIND>(S, s.K);
+  s.N = sensor_dirs(S, n);
   s.f = 0;
   s.cand0 = 0u;
   s.cand_any = 0;
@@ -362,9 +364,9 @@ IACT_DEV void store_record(float* r, const Sample& s) {
   r[0] = s.q.x;
   r[1] = s.q.y;
   r[2] = s.q.z;
-  r[3] = s.n.x;
-  r[4] = s.n.y;
-  r[5] = s.n.z;
+  r[3] = s.N.x;
+  r[4] = s.N.y;
+  r[5] = s.N.z;
   r[6] = s.nrwh;
   r[7] = s.G.kt;
   r[8] = s.G.ku;
@@ -393,11 +395,12 @@ IACT_DEV RecRaw fetch_record(const float4* r) {
 IACT_DEV Sample unpack_record(const RecRaw& q) {
   Sample s;
   s.q = mk3(q.a.x, q.a.y, q.a.z);
-  s.n = mk3(q.a.w, q.b.x, q.b.y);
+  s.N = mk3(q.a.w, q.b.x, q.b.y);
   s.nrwh = q.b.z;
   s.nrw = -2.f * s.nrwh;
   s.G = GridK{q.b.w, q.c.x, q.c.y};
   s.m = mk3(q.c.z, q.c.w, q.d.x);
+  s.n = -0.5f * s.m;  // exact: m = -2 n
   s.p = mk3(q.d.y, q.d.z, q.d.w);
   s.K = PlaneK{0.f, 0.f, 0.f};  // world hit coordinates are not traced from records
   s.f = 0;
@@ -610,51 +613,53 @@ IACT_DEV uint64_t far_run_shadow(const TraceArgs& A, const float* rec0, int n, i
 // the hit in sensor-grid coordinates (what the binning reads); XY also gives
 // the hit (x, y) in the sensor frame.
 template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = false,
-          bool RM = (OBST == 0), bool GEN = true, bool FAR = false>
+          bool GEN = true, bool FAR = false>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float qs,
                         float value, float& gu, float& gv, float& v, float& x, float& y,
                         float* vfac = nullptr) {
   // With m = -2 n, d.m = -2 (d.n) and value * (d.m) * nrwh = value * (d.n) *
-  // nrw, both exactly (scalings by powers of two).  RM false forms d.m from
-  // n, leaving m unread (traces with obstructions: their scalar registers are
-  // the scarcer resource).
-  float inv, c2, rs = 1.f;
-  f3 d, r;
-  // (every kernel takes this path when the scene has no further stage, so
-  // that all of them bin a ray alike)
+  // nrw, both exactly (scalings by powers of two).
+  float inv, c2, cu, rs = 1.f;
+  f3 d, u;
   if (SRC == IACT_SOURCE_POINT && (!STAGES || A.n_stages == 0)) {
     // The reflection of u = q + e, r = u + (u.m) n, is |u| times the unit
     // reflection, and the sensor hit depends only on its direction: u is not
     // normalised, 1/|u| enters the cosine and the parallel-ray test only
     // (the unit direction d is formed for the shadow test alone).  FAR: the
     // caller knows qs = 0 for every lane (u = e, loop-invariant).
-    const f3 u = FAR ? e : fma3(qs, s.q, e);
+    u = FAR ? e : fma3(qs, s.q, e);
     inv = frsq(dot(u, u));
-    const float c2u = RM ? dot(u, s.m) : -2.f * dot(u, s.n);
-    r = fma3(c2u, s.n, u);
-    c2 = c2u * inv;
+    cu = dot(u, s.m);
+    c2 = cu * inv;
     rs = inv;
     d = inv * u;
   } else {
     d = primary_direction<SRC>(s, e, qs, inv);
-    c2 = RM ? dot(d, s.m) : -2.f * dot(d, s.n);
-    r = fma3(c2, s.n, d);
+    cu = dot(d, s.m);
+    c2 = cu;
+    u = d;
   }
   // vfac: the value before the weight, v = vfac * nrwh without obstructions
   // or further stages
   if (vfac) *vfac = value * c2;
   v = value * c2 * s.nrwh;
   v *= primary_shadow<OBST>(A, s, d, active);
-  PlaneK K = s.K;
-  GridK G = s.G;
-  if (STAGES && A.n_stages > 0) {
+  bool ok;
+  if (!STAGES || A.n_stages == 0) {
+    // the hit of r = cu n + u from the sample's N and the direction's U
+    // (grid_hit; U is loop-invariant for far and parallel sources); every
+    // kernel bins a primary ray this way, so their pixel decisions agree
+    ok = grid_hit<GEN>(s.G, s.N, sensor_dirs(A.S, u), cu, gu, gv, rs);
+    if (XY) plane_xy<AXIS>(A.S, s.K, fma3(cu, s.n, u), x, y, rs);
+  } else {
+    f3 r = fma3(cu, s.n, u);
     f3 o = s.p;
     for (int k = 0; k < A.n_stages; ++k) reflect_at_stage(A, A.stages[k], o, r, v, active);
-    K = plane_consts(A.S, o);
-    G = grid_consts<KIND>(A.S, K);
+    const PlaneK K = plane_consts(A.S, o);
+    const GridK G = grid_consts<KIND>(A.S, K);
+    ok = plane_grid<KIND, AXIS, true>(A.S, G, r, gu, gv, 1.f);
+    if (XY) plane_xy<AXIS>(A.S, K, r, x, y, 1.f);
   }
-  const bool ok = plane_grid<KIND, AXIS, GEN || STAGES>(A.S, G, r, gu, gv, rs);
-  if (XY) plane_xy<AXIS>(A.S, K, r, x, y, rs);
   if (!active) v = 0.f;
   return ok && active;
 }
