@@ -286,28 +286,25 @@ IACT_DEV CylDir cyl_dir(f3 d, cptr<float> c) {
   D.ir = frcp(D.rda + eps);
   return D;
 }
-IACT_DEV float hit_cylinder_dir(f3 o, const CylDir& D, cptr<float> c) {
-  const float eps = 1e-8f;
-  const f3 p1 = mk3(c[0], c[1], c[2]);
-  const f3 ax = mk3(c[3], c[4], c[5]);
-  const float h = c[6], r = c[7];
-  const f3 oc = o - p1;
-  const float inf = __builtin_inff();
-  // Early out.  With w = d x ax, (oc.w)^2 / |w|^2 is the squared distance of
-  // the ray's line from the axis; when it exceeds r^2 by 2e-4 r^2 plus
-  // 1e-5 |oc|^2 (a bound on the rounding of this estimate and of the
-  // discriminant below, which has the same sign as r^2 minus that distance)
-  // no side or cap hit is possible.  Lines parallel to the axis (w = 0) and
-  // NaN geometry never take it.  The per-lane result is the miss for such a
-  // lane whatever its wave, and the rest of the test is skipped when it
-  // holds for the whole wave.
+// Early out.  With w = d x ax, (oc.w)^2 / |w|^2 is the squared distance of
+// the ray's line from the axis; when it exceeds r^2 by 2e-4 r^2 plus
+// 1e-5 |oc|^2 (a bound on the rounding of this estimate and of the
+// discriminant below, which has the same sign as r^2 minus that distance) no
+// side or cap hit is possible.  Lines parallel to the axis (w = 0) and NaN
+// geometry never take it.  The per-lane result is the miss for such a lane
+// whatever its wave, and the rest of the test is skipped when it holds for
+// the whole wave.
+IACT_DEV bool cyl_far(f3 oc, const CylDir& D, float r) {
   const float ow = dot(oc, D.w);
-  const bool far = ow * ow > D.ww * fmaf(1e-5f, dot(oc, oc), 1.0002f * r * r);
-  if (__all(far)) return inf;
-  const float oca = dot(oc, ax);
-  const f3 ocp = oc - oca * ax;
+  return ow * ow > D.ww * fmaf(1e-5f, dot(oc, oc), 1.0002f * r * r);
+}
+// The side and cap hits from the origin's axial split (oca = oc.ax,
+// ocp = oc - oca ax, cc = |ocp|^2 - r^2) and the direction terms.
+IACT_DEV float cyl_rest(f3 ocp, float oca, float cc, const CylDir& D, float h, float r,
+                        bool far) {
+  const float eps = 1e-8f;
+  const float inf = __builtin_inff();
   const float b = 2.f * dot(ocp, D.rdp);
-  const float cc = dot(ocp, ocp) - r * r;
   const float disc = b * b - 4.f * D.a * cc;
   const float sq = fsqrt(fmaxf(disc, 0.f));
   float t1 = (-b - sq) * D.ia, t2 = (-b + sq) * D.ia;
@@ -319,6 +316,35 @@ IACT_DEV float hit_cylinder_dir(f3 o, const CylDir& D, cptr<float> c) {
   tb = (tb > eps && dot(pb, pb) <= r * r) ? tb : inf;
   tt = (tt > eps && dot(pt, pt) <= r * r) ? tt : inf;
   return far ? inf : fminf(fminf(t1, t2), fminf(tb, tt));
+}
+IACT_DEV float hit_cylinder_dir(f3 o, const CylDir& D, cptr<float> c) {
+  const f3 p1 = mk3(c[0], c[1], c[2]);
+  const f3 ax = mk3(c[3], c[4], c[5]);
+  const float h = c[6], r = c[7];
+  const f3 oc = o - p1;
+  const bool far = cyl_far(oc, D, r);
+  if (__all(far)) return __builtin_inff();
+  const float oca = dot(oc, ax);
+  const f3 ocp = oc - oca * ax;
+  const float cc = dot(ocp, ocp) - r * r;
+  return cyl_rest(ocp, oca, cc, D, h, r, far);
+}
+// hit_cylinder_dir for two directions from one origin: the origin's terms
+// are formed once; each direction's result is hit_cylinder_dir's.
+IACT_DEV void hit_cylinder_dir2(f3 o, const CylDir& D0, const CylDir& D1, cptr<float> c,
+                                float& t0, float& t1) {
+  const f3 p1 = mk3(c[0], c[1], c[2]);
+  const f3 ax = mk3(c[3], c[4], c[5]);
+  const float h = c[6], r = c[7];
+  const f3 oc = o - p1;
+  const bool far0 = cyl_far(oc, D0, r), far1 = cyl_far(oc, D1, r);
+  t0 = t1 = __builtin_inff();
+  if (__all(far0 && far1)) return;
+  const float oca = dot(oc, ax);
+  const f3 ocp = oc - oca * ax;
+  const float cc = dot(ocp, ocp) - r * r;
+  if (!__all(far0)) t0 = cyl_rest(ocp, oca, cc, D0, h, r, far0);
+  if (!__all(far1)) t1 = cyl_rest(ocp, oca, cc, D1, h, r, far1);
 }
 IACT_DEV float hit_cylinder(f3 o, f3 d, cptr<float> c) {
   return hit_cylinder_dir(o, cyl_dir(d, c), c);

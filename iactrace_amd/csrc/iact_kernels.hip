@@ -924,13 +924,18 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         if (all_far && refined) {
           // far sources over a refined run: the candidates' shadow bits
           // first, then the run without shadow tests
+          f3 dsh[SPL];
 #pragma unroll
           for (int k = 0; k < SPL; ++k) {
             const f3 u = e[k];
-            const f3 d = frsq(dot(u, u)) * u;  // trace_ray's d of a far ray
-            blk[k] = far_run_shadow(A, (const float*)(recs + 4 * (size_t)jb), jend - jb,
-                                    rlist[k], rbits[k], -d);
+            dsh[k] = -(frsq(dot(u, u)) * u);  // trace_ray's -d of a far ray
           }
+          // (one chunk at a time: the two-direction walk of the adjoint
+          // costs this kernel two waves per SIMD of occupancy)
+          const float* rec0 = (const float*)(recs + 4 * (size_t)jb);
+#pragma unroll
+          for (int k = 0; k < SPL; ++k)
+            blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k]);
           run(F_{}, T_{}, T_{}, T_{});
         } else if (all_far) {
           run(T_{}, T_{}, T_{}, F_{});
@@ -1197,10 +1202,17 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
         if (all_far && refined) {
           // the forward's shadow bits of the run (far_run_shadow), then the
           // run without shadow tests
+          f3 dsh[SPL];
 #pragma unroll
-          for (int k = 0; k < SPL; ++k)
-            blk[k] = far_run_shadow(A, (const float*)(recs + 4 * (size_t)jb), jend - jb,
-                                    rlist[k], rbits[k], -fd[k]);
+          for (int k = 0; k < SPL; ++k) dsh[k] = -fd[k];
+          const float* rec0 = (const float*)(recs + 4 * (size_t)jb);
+          if constexpr (SPL == 2) {
+            far_run_shadow2(A, rec0, jend - jb, rlist, rbits, dsh, blk);
+          } else {
+#pragma unroll
+            for (int k = 0; k < SPL; ++k)
+              blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k]);
+          }
           run(F_{}, T_{}, T_{}, T_{});
         } else if (all_far) {
           run(T_{}, T_{}, T_{}, F_{});

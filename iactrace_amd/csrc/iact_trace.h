@@ -602,6 +602,63 @@ IACT_DEV uint64_t far_run_shadow(const TraceArgs& A, const float* rec0, int n, i
   return blocked;
 }
 
+// far_run_shadow for a lane's two sources (two source chunks) at once: the
+// union of both chunks' candidates is walked once and each candidate is
+// tested along both directions, sharing the sample's terms (the origin's
+// part of the cylinder test, the box's slab offsets).  A candidate tested for
+// the other chunk's rays is an obstruction like any other: a ray it blocks
+// is blocked, so the bits are those of two far_run_shadow calls.
+IACT_DEV void far_run_shadow2(const TraceArgs& A, const float* rec0, int n, const int* list,
+                              const uint64_t* bits, const f3* dsh, uint64_t* blk) {
+  const iact_obstructions& ob = A.ob;
+  const int lane = lane_id();
+  const float* pr = rec0 + (size_t)IACT_REC_WORDS * (lane < n ? lane : 0) + 13;
+  const f3 myp = mk3(pr[0], pr[1], pr[2]);
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes;
+  const bool in0 = (bits[0] >> lane) & 1ull;
+  uint64_t b0 = 0ull, b1 = 0ull;
+#pragma unroll
+  for (int k0 = 0; k0 < 2; ++k0) {
+    uint64_t bb = bits[k0];
+    while (bb != 0ull) {
+      const int pos = __builtin_ctzll(bb);
+      bb &= bb - 1ull;
+      const int j = __builtin_amdgcn_readlane(list[k0], pos);
+      // (a candidate of both chunks is walked with the first)
+      if (k0 == 1 && __ballot(in0 && list[0] == j) != 0ull) continue;
+      if (j < c0) {
+        const cptr<float> c = as_const(ob.cylinders) + 8 * j;
+        const CylDir D0 = cyl_dir(dsh[0], c), D1 = cyl_dir(dsh[1], c);
+        for (int i = 0; i < n; ++i) {
+          if (__all(((b0 >> i) & (b1 >> i) & 1ull) != 0ull)) continue;
+          float t0, t1;
+          hit_cylinder_dir2(lane_value(myp, i), D0, D1, c, t0, t1);
+          if (t0 < 1e10f) b0 |= 1ull << i;
+          if (t1 < 1e10f) b1 |= 1ull << i;
+        }
+      } else if (j < c1) {
+        const cptr<float> b = as_const(ob.boxes) + 6 * (j - c0);
+        const BoxDir D0 = box_dir(dsh[0]), D1 = box_dir(dsh[1]);
+        for (int i = 0; i < n; ++i) {
+          if (__all(((b0 >> i) & (b1 >> i) & 1ull) != 0ull)) continue;
+          const f3 o = lane_value(myp, i);
+          if (hit_box_dir(o, D0, b) < 1e10f) b0 |= 1ull << i;
+          if (hit_box_dir(o, D1, b) < 1e10f) b1 |= 1ull << i;
+        }
+      } else {
+        for (int i = 0; i < n; ++i) {
+          if (__all(((b0 >> i) & (b1 >> i) & 1ull) != 0ull)) continue;
+          const f3 o = lane_value(myp, i);
+          if (hit_obstruction(ob, j, o, dsh[0]) < 1e10f) b0 |= 1ull << i;
+          if (hit_obstruction(ob, j, o, dsh[1]) < 1e10f) b1 |= 1ull << i;
+        }
+      }
+    }
+  }
+  blk[0] = b0;
+  blk[1] = b1;
+}
+
 // Trace one (source, sample) ray to the sensor plane (render.py:118-157).
 // GEN false (image traces of launches whose samples all satisfy
 // plane_test_redundant, on square pixels) leaves out the parallel-ray test
