@@ -399,11 +399,13 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
     grid = (A.sm_items + wpb - 1) / wpb;
     if (!(L && L->blocks_per_cu)) {
       // every block ends with a flush of its image (an fp64 atomic per lit
-      // pixel): by default a block gets at least 8 items per wave (measured
-      // best from 256 to 16384 C3 sources), from 4 to 64 blocks per CU
-      const long long per8 = (A.sm_items + 8ll * wpb - 1) / (8ll * wpb);
+      // pixel) or starts by loading the image gradient: by default a block
+      // gets at least 16 items per wave (measured best or within noise from
+      // 256 to 16384 C3 sources, forward and adjoint), from 4 to 64 blocks
+      // per CU
+      const long long per = (A.sm_items + 16ll * wpb - 1) / (16ll * wpb);
       const long long lo = 4ll * dev.n_cu;
-      grid = per8 < lo ? (grid < lo ? grid : lo) : per8;
+      grid = per < lo ? (grid < lo ? grid : lo) : per;
     }
   } else if (mode == IACT_MODE_RESPONSE) {
     grid = ns;
