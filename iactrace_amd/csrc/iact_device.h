@@ -749,19 +749,6 @@ IACT_DEV void hex_round(float q, float r, float& qi_out, float& ri_out) {
   ri_out = fix_r ? -qi - si : ri;
 }
 
-// Nearest hex centre by rounding q and r and moving at most one step along
-// the larger remainder (J. Rus's axial rounding): the cell of hex_round
-// except for points within rounding of a cell boundary, in 13 instead of 15
-// vector instructions.
-IACT_DEV void hex_round_nearest(float q, float r, float& qi_out, float& ri_out) {
-  const float qg = rintf(q), rg = rintf(r);
-  const float xr = q - qg, yr = r - rg;
-  const bool c = fabsf(xr) >= fabsf(yr);
-  const float t = rintf(c ? fmaf(0.5f, yr, xr) : fmaf(0.5f, xr, yr));
-  qi_out = c ? qg + t : qg;
-  ri_out = c ? rg : rg + t;
-}
-
 // Table lookup with bounds (hexagonal.py:155-172); -1 when off-table/empty.
 IACT_DEV int hex_lookup(const DevSensor& S, const int32_t* table, float qi, float ri) {
   const float qa = qi - S.qmin_f, ra = ri - S.rmin_f;
@@ -786,19 +773,11 @@ IACT_DEV int hex_pixel_grid(const DevSensor& S, const int32_t* table, float q, f
   return (pid >= 0 && !(hd > S.edge_thr)) ? pid : -1;
 }
 
-// float -> int32 with the hardware's saturation (v_cvt_i32_f32): out-of-range
-// values clamp to INT_MIN / INT_MAX and NaN converts to 0, where a C cast
-// would be undefined.
-IACT_DEV int trunc_i32(float x) {
-  int r;
-  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-  return r;
-}
 
 // Hard hex binning of a traced ray's hit (table-relative fractional axial
 // q, r), -1 when the trace rejected the ray (ok false): the pixel of
 // hex_pixel_grid (up to points within rounding of a cell boundary, see
-// hex_round_nearest), with the range test as two unsigned compares, the table
+// hex_cell_floor), with the range test as two unsigned compares, the table
 // read for every lane (the index clamped into the table) and then selected,
 // and the edge test only for sensors with an edge.  Without one
 // (edge_thr = 1) the reference's test can only drop a point whose hex norm
