@@ -334,9 +334,7 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
   A.acc = (L && !A.det) ? L->accumulate : 0;
   if (A.acc < 0 || A.acc > 2) return set_error(IACT_ERR_INVALID, "accumulate mode %d", A.acc);
   const int P = sc.sensor.n_pixels;
-  const size_t tab = sc.sensor.kind == IACT_SENSOR_HEX
-                         ? (size_t)sc.sensor.table_q * sc.sensor.table_r * sizeof(int32_t)
-                         : 0;
+  const size_t tab = hex_table_bytes(sc.sensor);
   size_t budget = (size_t)(dev.lds_per_block > 0 ? dev.lds_per_block : 65536);
   if (budget > 65536) budget = 65536;
   if (tab > budget / 2)
@@ -622,8 +620,7 @@ int32_t iact_sensor_accumulate(const iact_sensor* sensor, const float* x, const 
   if (n < 0) return set_error(IACT_ERR_INVALID, "negative point count");
   if (!out) return set_error(IACT_ERR_INVALID, "null output");
   if (n > 0 && (!x || !y || !v)) return set_error(IACT_ERR_INVALID, "null point buffers");
-  if (sensor->kind == IACT_SENSOR_HEX &&
-      (size_t)sensor->table_q * sensor->table_r * sizeof(int32_t) > 32768)
+  if (hex_table_bytes(*sensor) > 32768)
     return set_error(IACT_ERR_UNSUPPORTED, "hex lookup table exceeds 32 KiB of LDS");
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = launch_fill(out, sensor->n_pixels, 0.f, st);

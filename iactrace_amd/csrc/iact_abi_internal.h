@@ -70,5 +70,11 @@ int validate_sensor(const iact_sensor& S);
 DevSensor make_dev_sensor(const iact_sensor& s);
 void fill_trace_args(TraceArgs& A, const iact_scene& sc, const double* sources,
                      const float* values, int ns);
+// LDS bytes of a hexagonal sensor's staged lookup table (the table with a
+// border entry on every side, stage_hex_table); 0 for square sensors.
+inline size_t hex_table_bytes(const iact_sensor& s) {
+  return s.kind == IACT_SENSOR_HEX ? (size_t)(s.table_q + 2) * (s.table_r + 2) * sizeof(int32_t)
+                                   : 0;
+}
 
 }  // namespace iact

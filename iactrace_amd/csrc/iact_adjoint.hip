@@ -265,10 +265,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_adj(AdjArgs B) {
   extern __shared__ float smem[];
   const int P = A.S.n_pixels;
   float* G = smem;
-  int32_t* tab = (int32_t*)(smem + P);
   for (int i = threadIdx.x; i < P; i += blockDim.x) G[i] = B.grad_image[i];
-  if (KIND == IACT_SENSOR_HEX)
-    for (int i = threadIdx.x; i < A.S.tq * A.S.tr; i += blockDim.x) tab[i] = A.S.table[i];
+  const int32_t* tab = KIND == IACT_SENSOR_HEX ? stage_hex_table(A.S, (int32_t*)(smem + P))
+                                               : (const int32_t*)(smem + P);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
@@ -673,10 +672,8 @@ int32_t iact_trace_bwd(const iact_scene* scene, const double* sources, const flo
     if (lay.end > 0 && (!workspace || workspace_bytes < lay.end))
       return set_error(IACT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", lay.end,
                        workspace_bytes);
-    const size_t tab = sc.sensor.kind == IACT_SENSOR_HEX
-                           ? (size_t)sc.sensor.table_q * sc.sensor.table_r * sizeof(int32_t)
-                           : 0;
-    shp.lds = (size_t)sc.sensor.n_pixels * sizeof(float) + tab;
+    const size_t tab = hex_table_bytes(sc.sensor);
+    shp.lds = (size_t)(sc.sensor.n_pixels + 1) * sizeof(float) + tab;  // G[-1] = 0 + G
     if (shp.lds > 65536)
       return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS",
                        shp.lds);

@@ -413,9 +413,7 @@ int32_t iact_trace_batched_bwd(const iact_scene* scene, const iact_batch* batch,
     return set_error(IACT_ERR_UNSUPPORTED,
                      "the adjoint kernel covers primary-only scenes (stage>=1 mirrors present)");
   // the adjoint stages one instance's image gradient (+ hex table) in LDS
-  const size_t tab = scene->sensor.kind == IACT_SENSOR_HEX
-                         ? (size_t)scene->sensor.table_q * scene->sensor.table_r * sizeof(int32_t)
-                         : 0;
+  const size_t tab = hex_table_bytes(scene->sensor);
   shp.lds = (size_t)scene->sensor.n_pixels * sizeof(float) + tab;
   if (shp.lds > 65536)
     return set_error(IACT_ERR_UNSUPPORTED, "image gradient + tables need %zu bytes of LDS",

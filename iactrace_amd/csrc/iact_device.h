@@ -749,11 +749,30 @@ IACT_DEV void hex_round(float q, float r, float& qi_out, float& ri_out) {
   ri_out = fix_r ? -qi - si : ri;
 }
 
+// The hexagonal lookup table as every kernel reads it: a copy in LDS with a
+// border of -1 entries around the sensor's tq x tr entries (rows of tr + 2),
+// so that a cell index clamped into [-1, tq] x [-1, tr] reads -1 off the
+// table, and with every entry below -1 stored as -1 (the reference's empty
+// marker): cell (q, r) sits at row q + 1, column r + 1.  The block copies it
+// from S.table and must synchronise before reading.
+__host__ __device__ inline int hex_table_words(const DevSensor& S) {
+  return (S.tq + 2) * (S.tr + 2);
+}
+IACT_DEV const int32_t* stage_hex_table(const DevSensor& S, int32_t* lds) {
+  const int w = S.tr + 2, n = hex_table_words(S);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int a = i / w - 1, b = i % w - 1;
+    lds[i] = (a >= 0 && a < S.tq && b >= 0 && b < S.tr) ? max(S.table[a * S.tr + b], -1) : -1;
+  }
+  return lds;
+}
+
 // Table lookup with bounds (hexagonal.py:155-172); -1 when off-table/empty.
+// table: the staged copy (stage_hex_table).
 IACT_DEV int hex_lookup(const DevSensor& S, const int32_t* table, float qi, float ri) {
   const float qa = qi - S.qmin_f, ra = ri - S.rmin_f;
   if (!(qa >= 0.f && qa < S.tq_f && ra >= 0.f && ra < S.tr_f)) return -1;
-  return table[(int)qa * S.tr + (int)ra];
+  return table[((int)qa + 1) * (S.tr + 2) + (int)ra + 1];
 }
 
 // Hexagonal norm (hexagonal.py:42-47) of the offset from the cell centre,
@@ -777,8 +796,7 @@ IACT_DEV int hex_pixel_grid(const DevSensor& S, const int32_t* table, float q, f
 // Hard hex binning of a traced ray's hit (table-relative fractional axial
 // q, r), -1 when the trace rejected the ray (ok false): the pixel of
 // hex_pixel_grid (up to points within rounding of a cell boundary, see
-// hex_cell_floor), with the range test as two unsigned compares, the table
-// read for every lane (the index clamped into the table) and then selected,
+// hex_cell_staged), the staged table read for every lane and then selected,
 // and the edge test only for sensors with an edge.  Without one
 // (edge_thr = 1) the reference's test can only drop a point whose hex norm
 // rounds above 1, i.e. a point within rounding of a cell boundary.
@@ -788,27 +806,33 @@ IACT_DEV int hex_pixel_grid(const DevSensor& S, const int32_t* table, float q, f
 // triangle those lines cut out the centre is ((A - C + 1) div 3,
 // (C - B + 1) div 3) for A, B, C the floors (checked against cube rounding on
 // the CPU).  k div 3 is floor(k/3 + 1/2) for an integer k: 1/6 clear of
-// every integer, so the fp32 product is safe.  The cell of hex_round except
-// for points within rounding of a cell boundary; 12 vector instructions
-// with the conversions to int.
-IACT_DEV void hex_cell_floor(float q, float r, int& iq, int& ir) {
+// every integer, so the fp32 product is safe, and so is floor(k/3 + 3/2),
+// the staged table's row (cell + 1).  Clamped into [0, tq + 1] x [0, tr + 1]
+// before the conversion (a med3 each; NaN converts to 0), every cell off the
+// sensor's table lands on the staged border (-1), so no range test is needed.
+// The cell of hex_round except for points within rounding of a cell
+// boundary; 12 vector instructions with the clamps and conversions.
+IACT_DEV void hex_cell_staged(const DevSensor& S, float q, float r, int& row, int& col) {
   const float A = floorf(fmaf(2.f, q, r));
   const float B = floorf(fmaf(-2.f, r, -q));
   const float C = floorf(r - q);
-  iq = floor_i32(fmaf(A - C, 1.f / 3.f, 0.5f));
-  ir = floor_i32(fmaf(C - B, 1.f / 3.f, 0.5f));
+  row = floor_i32(__builtin_amdgcn_fmed3f(fmaf(A - C, 1.f / 3.f, 1.5f), 0.f, S.tq_f + 1.f));
+  col = floor_i32(__builtin_amdgcn_fmed3f(fmaf(C - B, 1.f / 3.f, 1.5f), 0.f, S.tr_f + 1.f));
 }
 
 template <bool EDGE = true>
 IACT_DEV int hex_pixel_hit(const DevSensor& S, const int32_t* table, float q, float r, bool ok) {
-  int iq, ir;
-  hex_cell_floor(q, r, iq, ir);
-  const float qi = (float)iq, ri = (float)ir;
-  const bool in = ((unsigned)iq < (unsigned)S.tq) & ((unsigned)ir < (unsigned)S.tr);
-  const unsigned last = __umul24((unsigned)S.tq, (unsigned)S.tr) - 1u;
-  const int pid = table[min(__umul24((unsigned)iq, (unsigned)S.tr) + (unsigned)ir, last)];
-  bool good = ok & in & (pid >= 0);
-  if (EDGE && S.edge) good = good & !(hex_norm_axial(q - qi, r - ri) > S.edge_thr);
+  int row, col;
+  hex_cell_staged(S, q, r, row, col);
+  // the entries are -1 or pixel ids; the index is kept whole (an empty asm)
+  // so that it stays one multiply-add, which the address then shifts
+  unsigned idx = __umul24((unsigned)row, (unsigned)(S.tr + 2)) + (unsigned)col;
+  asm volatile("" : "+v"(idx));
+  const int pid = table[idx];
+  bool good = ok;
+  // (a clamped cell reads -1, whatever the edge test says)
+  if (EDGE && S.edge)
+    good = good & !(hex_norm_axial(q - (float)(row - 1), r - (float)(col - 1)) > S.edge_thr);
   return good ? pid : -1;
 }
 

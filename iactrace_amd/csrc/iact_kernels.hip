@@ -429,9 +429,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_accumulate(DevSensor S, cons
                                                                long long n, float* out,
                                                                int32_t* pixel) {
   extern __shared__ float smem[];
-  int32_t* tab = (int32_t*)smem;
+  const int32_t* tab = (const int32_t*)smem;
   if (KIND == IACT_SENSOR_HEX) {
-    for (int i = threadIdx.x; i < S.tq * S.tr; i += blockDim.x) tab[i] = S.table[i];
+    tab = stage_hex_table(S, (int32_t*)smem);
     __syncthreads();
   }
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -457,7 +457,7 @@ template <int KIND, int SPLAT>
 static hipError_t launch_acc_t(const DevSensor& S, const float* x, const float* y,
                                const float* v, long long n, float* out, int32_t* pixel,
                                hipStream_t st) {
-  const size_t lds = KIND == IACT_SENSOR_HEX ? (size_t)S.tq * S.tr * sizeof(int32_t) : 0;
+  const size_t lds = KIND == IACT_SENSOR_HEX ? (size_t)hex_table_words(S) * sizeof(int32_t) : 0;
   long long g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   if (g < 1) g = 1;
@@ -496,12 +496,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace(TraceArgs A) {
     const int n = (det ? wpb : 1) * P;
     for (int i = threadIdx.x; i < n; i += blockDim.x) img_lds[i] = 0.f;
   }
-  if (KIND == IACT_SENSOR_HEX && A.lds_table) {
-    const int n = A.S.tq * A.S.tr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
-  }
+  // hex tables are always staged in LDS
+  const int32_t* table =
+      KIND == IACT_SENSOR_HEX && A.lds_table ? stage_hex_table(A.S, tab_lds) : tab_lds;
   __syncthreads();
-  const int32_t* table = tab_lds;  // hex tables are always staged in LDS
   float* img = img_lds + (det ? wave * P : 0);
 
   int run_pix = -1;
@@ -742,8 +740,13 @@ IACT_DEV bool refine_candidates(const TraceArgs& A, int f, int c, int blk, int l
 // candidate for any of the lane's chunks (most of them when culling), PART 2
 // those with one, with shadow tests; the host launches both over the same
 // items, so each kernel's registers are allocated for its own loop.
+// PART 2 is held to 72 VGPRs (7 waves per SIMD): its shadow tests wait on
+// table loads, and the extra waves hide more of that than the few spills
+// (outside the per-ray loops) cost -- measured 10 % faster on the alignment
+// fit's instances than at its natural 84 VGPRs (5 waves); 8 waves lose 40 %.
 template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL, int PART>
-__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
+__global__ __launch_bounds__(IACT_MAX_BLOCK)
+__attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArgs A) {
   extern __shared__ float smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
@@ -752,12 +755,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
   int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? P : 0));
   if (A.lds_image)
     for (int i = threadIdx.x; i < P; i += blockDim.x) img[i] = 0.f;
-  if (KIND == IACT_SENSOR_HEX && A.lds_table) {
-    const int n = A.S.tq * A.S.tr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
-  }
+  const int32_t* table =
+      KIND == IACT_SENSOR_HEX && A.lds_table ? stage_hex_table(A.S, tab_lds) : tab_lds;
   __syncthreads();
-  const int32_t* table = tab_lds;
   // pixel adds go to the LDS image or, when it does not fit, to global
   // memory -- two separate pointers, so each add is a plain LDS or global
   // atomic (a pointer that may be either would make it a flat atomic, which
@@ -797,6 +797,9 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     const float4* recs = (const float4*)A.recs;
     int run_pix[SPL];
     float run_v[SPL];
+    // far runs (FR) sum cu nrwh over the facet (run_l) and take the facet's
+    // 1/|u| (run_s) once, when the run is added or the facet ends
+    float run_l[SPL], run_s[SPL];
     // the current facet's obstruction candidates (per chunk of 64 sources),
     // their refinement for the sample block, and each lane source's offset
     FacetCull fc[SPL];
@@ -810,6 +813,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     for (int k = 0; k < SPL; ++k) {
       run_pix[k] = -1;
       run_v[k] = 0.f;
+      run_l[k] = 0.f;
+      run_s[k] = 1.f;
       blk[k] = 0ull;
     }
     // one sample's rays, with shadow tests (SH) or without; the shadow
@@ -833,16 +838,17 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
           smp.rlist = rlist[k];
           smp.rbits = rbits[k];
         }
-        float gu, gv, v, x, y, vf;
+        float gu, gv, v, x, y, vf, cu;
         // a ray's value is linear in its source's: hard binning traces unit
         // values and scales a run once, when it is added
         const bool ok = trace_ray<SRC, KIND, STAGES, AXIS, OB, false, GEN, FR>(
-            A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf);
+            A, smp, true, e[k], qs[k], SPLAT ? value[k] : 1.f, gu, gv, v, x, y, &vf, &cu);
         if (SB) {
           // the run's shadow bits (far_run_shadow) in place of the test
           const float sb = ((blk[k] >> bi) & 1ull) ? 0.f : 1.f;
           v *= sb;
           vf *= sb;
+          cu *= sb;
         }
         if (!SPLAT) {
           // run of equal pixels per lane and source (consecutive samples of
@@ -850,15 +856,19 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
           // when it ends
           const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
           if (pix != run_pix[k]) {
-            const float rv = value[k] * run_v[k];
+            const float rv = value[k] * fmaf(run_s[k], run_l[k], run_v[k]);
             if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
             run_pix[k] = pix;
             run_v[k] = 0.f;
+            run_l[k] = 0.f;
           }
           // a run of pix = -1 is never added; without shadow tests and
-          // stages v = vf * nrwh, summed in one fma
+          // stages v = vf * nrwh, summed in one fma (far runs: v = cu nrwh
+          // run_s)
           if (OB || STAGES)
             run_v[k] += v;
+          else if (SRC == IACT_SOURCE_POINT && FR)
+            run_l[k] = fmaf(cu, smp.nrwh, run_l[k]);
           else
             run_v[k] = fmaf(vf, smp.nrwh, run_v[k]);
         } else {
@@ -901,6 +911,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
         e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
         qs[k] = source_qs(A, e[k]);
         near |= qs[k] != 0.f;
+        run_s[k] = frsq(dot(e[k], e[k]));  // trace_ray's 1/|u| of a far ray
       }
       const bool all_far = SRC == IACT_SOURCE_POINT && __ballot(near) == 0ull;
       // the next record is loaded while one is traced (the record table has
@@ -914,6 +925,13 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
           const RecRaw nxt = fetch_record(rp);
           trace_one(sh, gen, far, sbits, cur, j - jb);
           cur = nxt;
+        }
+        // the facet's far sums join the run (which may go on into the next
+        // facet)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+          run_v[k] = fmaf(run_s[k], run_l[k], run_v[k]);
+          run_l[k] = 0.f;
         }
       };
       using T_ = std::true_type;
@@ -954,7 +972,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
     if (!SPLAT) {
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
-        const float rv = value[k] * run_v[k];
+        const float rv = value[k] * fmaf(run_s[k], run_l[k], run_v[k]);
         if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
       }
     }
@@ -988,20 +1006,25 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_sm(TraceArgs A) {
 // fp64 per-facet sums go to k_torque_to_euler, which dots them with the axes.
 // Source and value gradients are summed per lane (per source) over an item's
 // samples.
+// Held to 80 VGPRs (6 waves per SIMD), for the reason PART 2 of k_trace_sm
+// is: measured 3 % (PART 1, natural 82 VGPRs) and 9 % (PART 2, natural 112)
+// faster on the alignment fit's instances; 7 waves for PART 2 lose 8 %.
 template <int SRC, int KIND, bool AXIS, int OBST, int SPL, int PART>
-__global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
+__global__ __launch_bounds__(IACT_MAX_BLOCK)
+__attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
   const TraceArgs& A = B.T;
   extern __shared__ float smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const int P = A.S.n_pixels;
-  float* G = smem;
-  int32_t* tab_lds = (int32_t*)(smem + P);
+  // the image gradient behind one zero entry: a dropped ray (pixel -1)
+  // reads G[-1] = 0
+  float* G = smem + 1;
+  int32_t* tab_lds = (int32_t*)(smem + 1 + P);
+  if (threadIdx.x == 0) smem[0] = 0.f;
   for (int i = threadIdx.x; i < P; i += blockDim.x) G[i] = B.grad_image[i];
-  if (KIND == IACT_SENSOR_HEX)
-    for (int i = threadIdx.x; i < A.S.tq * A.S.tr; i += blockDim.x) tab_lds[i] = A.S.table[i];
+  const int32_t* table = KIND == IACT_SENSOR_HEX ? stage_hex_table(A.S, tab_lds) : tab_lds;
   __syncthreads();
-  const int32_t* table = tab_lds;
   const long long total = (long long)A.nf * A.M;
   const bool lean = as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u &&
                     !A.S.edge && (KIND != IACT_SENSOR_SQUARE || A.S.v_scale == 1.f);
@@ -1042,19 +1065,26 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
     // d / 2), are below 2^-30 * 1.5 * sum |b| (|q| inv <= 2^-30, |c2| <= 2):
     // some 40 times below the fp32 rounding of the sum N x d that they join
     // (also per ray, in the reference's fp32 gradient), and are left out.
+    // With b = a nrw = -2 a nrwh and n = -m / 2 the lane sums a nrwh m, which
+    // is b n exactly (scalings by powers of two), from the record's m and
+    // nrwh words, and fC holds -C / 2; both leave out the source's value
+    // (a = G shadow value), which multiplies them at the run's end.
     f3 fd[SPL], fN[SPL];
     float finv[SPL], fC[SPL];
     uint64_t blk[SPL];  // shadow bits of a far run (far_run_shadow)
     const bool want_pc = want_pos || want_src;
     // one sample's rays (SPL per lane): the forward's trace up to the pixel,
     // then the lane's share of the sample's torque and dL/dp (tq, gp), or
-    // (FR) the far sums
-    auto adjoint_one = [&](auto sh, auto gen, auto far, auto sbits, const RecRaw& rec, int bi,
-                           f3& tq, f3& gp) {
+    // (FR) the far sums.  GX false: the launch wants no value, source or
+    // position gradients (rotations only), and the loop has no code for them.
+    auto adjoint_one = [&](auto sh, auto gen, auto far, auto sbits, auto gx, const RecRaw& rec,
+                           int bi, f3& tq, f3& gp) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
       constexpr bool GEN = decltype(gen)::value;
       constexpr bool FR = decltype(far)::value;
       constexpr bool SB = decltype(sbits)::value;
+      constexpr bool GX = decltype(gx)::value;
+      const bool wval = GX && want_val, wsrc = GX && want_src, wpc = GX && want_pc;
       const Sample smp0 = unpack_record(rec);
       float san_x = 0.f, san_y = 0.f, san_z = 0.f, st = 0.f;
       float sw_x = 0.f, sw_y = 0.f, sw_z = 0.f;
@@ -1092,18 +1122,19 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
         float gu, gv;
         const bool ok = grid_hit<GEN>(smp.G, smp.N, sensor_dirs(A.S, u), c2u, gu, gv, rs);
         const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
-        const float gp_pix = pix >= 0 ? G[pix] : 0.f;
+        const float gp_pix = G[pix];  // 0 for pix = -1
         // a = G value shadow; dc = a nrw is applied per sample
-        const float a = (OB || SB) ? gp_pix * value[k] * shadow : gp_pix * value[k];
-        if (want_val) {
+        const float gs = (OB || SB) ? gp_pix * shadow : gp_pix;
+        const float a = gs * value[k];
+        if (wval) {
           // dL/dvalue = G v / value = G shadow (d.m) nrwh
-          const float gs = (OB || SB) ? gp_pix * shadow : gp_pix;
           gval[k] = fmaf(gs * (c2u * inv), smp.nrwh, gval[k]);
         }
         if (SRC == IACT_SOURCE_POINT && FR) {
-          const float b = a * smp.nrw;
-          fN[k] = fma3(b, smp.n, fN[k]);
-          if (want_pc) fC[k] = fmaf(b, c2u * inv, fC[k]);
+          // b n = bm m; the source's value scales the run's sums once
+          const float bm = gs * smp.nrwh;
+          fN[k] = fma3(bm, smp.m, fN[k]);
+          if (wpc) fC[k] = fmaf(bm, c2u * inv, fC[k]);
         } else if (SRC == IACT_SOURCE_POINT) {
           const float k1 = a * inv;  // dc / |u| before nrw
           san_x = fmaf(k1, u.x, san_x);
@@ -1115,7 +1146,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
           sw_x = fmaf(k2, u.x, sw_x);
           sw_y = fmaf(k2, u.y, sw_y);
           sw_z = fmaf(k2, u.z, sw_z);
-          if (want_src) {
+          if (wsrc) {
             // dL/dsrc = -dL/du = -(dc / |u|) (n - (d.n) d)
             const float tsrc = k1 * smp.nrw;
             const float cdn = -0.5f * c2u * inv;
@@ -1126,7 +1157,7 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
           san_x = fmaf(a, d.x, san_x);
           san_y = fmaf(a, d.y, san_y);
           san_z = fmaf(a, d.z, san_z);
-          if (want_src) {
+          if (wsrc) {
             // d = the source vector as given: dL/dsrc = dc n
             gsrc[k] = fma3(a * smp.nrw, smp.n, gsrc[k]);
           }
@@ -1183,18 +1214,24 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
       RecRaw cur = fetch_record(rp);
       f3 T = mk3(0.f, 0.f, 0.f), Gp = mk3(0.f, 0.f, 0.f);
       const int jb = j;
-      auto run = [&](auto sh, auto gen, auto far, auto sbits) {
+      auto run_gx = [&](auto sh, auto gen, auto far, auto sbits, auto gx) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
           f3 tq, gp;
-          adjoint_one(sh, gen, far, sbits, cur, j - jb, tq, gp);
+          adjoint_one(sh, gen, far, sbits, gx, cur, j - jb, tq, gp);
           if (!decltype(far)::value) {
             T = T + tq;
             Gp = Gp + gp;
           }
           cur = nxt;
         }
+      };
+      auto run = [&](auto sh, auto gen, auto far, auto sbits) {
+        if (want_val || want_pc)
+          run_gx(sh, gen, far, sbits, std::true_type{});
+        else
+          run_gx(sh, gen, far, sbits, std::false_type{});
       };
       using T_ = std::true_type;
       using F_ = std::false_type;
@@ -1230,9 +1267,10 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_adj_sm(AdjArgs B) {
       if (all_far) {
 #pragma unroll
         for (int k = 0; k < SPL; ++k) {
-          T = T + cross(fN[k], fd[k]);
+          const f3 Nk = value[k] * fN[k];
+          T = T + cross(Nk, fd[k]);
           if (want_pc) {
-            const f3 gpk = finv[k] * fma3(0.5f * fC[k], fd[k], fN[k]);
+            const f3 gpk = finv[k] * fma3(-value[k] * fC[k], fd[k], Nk);  // fC = -C / 2
             Gp = Gp + gpk;
             if (want_src) gsrc[k] = gsrc[k] - gpk;
           }
@@ -1292,11 +1330,8 @@ __global__ __launch_bounds__(IACT_MAX_BLOCK) void k_trace_response(TraceArgs A) 
   const int P = A.S.n_pixels;
   float* row = smem;
   int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? P : 0));
-  if (KIND == IACT_SENSOR_HEX && A.lds_table) {
-    const int n = A.S.tq * A.S.tr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tab_lds[i] = A.S.table[i];
-  }
-  const int32_t* table = tab_lds;
+  const int32_t* table =
+      KIND == IACT_SENSOR_HEX && A.lds_table ? stage_hex_table(A.S, tab_lds) : tab_lds;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const long long total = (long long)A.nf * A.M;

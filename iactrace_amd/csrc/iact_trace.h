@@ -673,7 +673,7 @@ template <int SRC, int KIND, bool STAGES, bool AXIS, int OBST = 2, bool XY = fal
           bool GEN = true, bool FAR = false>
 IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, float qs,
                         float value, float& gu, float& gv, float& v, float& x, float& y,
-                        float* vfac = nullptr) {
+                        float* vfac = nullptr, float* cup = nullptr) {
   // With m = -2 n, d.m = -2 (d.n) and value * (d.m) * nrwh = value * (d.n) *
   // nrw, both exactly (scalings by powers of two).
   float inv, c2, cu, rs = 1.f;
@@ -699,6 +699,7 @@ IACT_DEV bool trace_ray(const TraceArgs& A, const Sample& s, bool active, f3 e, 
   // vfac: the value before the weight, v = vfac * nrwh without obstructions
   // or further stages
   if (vfac) *vfac = value * c2;
+  if (cup) *cup = cu;  // vfac = value cu rs for point sources
   v = value * c2 * s.nrwh;
   v *= primary_shadow<OBST>(A, s, d, active);
   bool ok;

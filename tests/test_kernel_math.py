@@ -55,43 +55,61 @@ def test_cylinder_early_out_never_skips_a_hit():
     assert not np.any(far & hit64g)
 
 
-def _hex_cell_floor(q, r):
-    """hex_cell_floor (iact_device.h) in fp32: the centre from the floors of
-    the three cube-coordinate differences."""
+def _hex_cell_staged(q, r, tq, tr):
+    """hex_cell_staged (iact_device.h) in fp32: the centre from the floors of
+    the three cube-coordinate differences, as the staged table's row and
+    column (cell + 1) clamped into [0, tq + 1] x [0, tr + 1] before the
+    conversion."""
     f = np.float32
     q, r = q.astype(f), r.astype(f)
     A = np.floor(f(2) * q + r)
     B = np.floor(-q - f(2) * r)
     C = np.floor(r - q)
     third = f(1.0 / 3.0)
-    return (np.floor((A - C) * third + f(0.5)).astype(np.int64),
-            np.floor((C - B) * third + f(0.5)).astype(np.int64))
+    row = np.clip((A - C) * third + f(1.5), f(0), f(tq + 1))
+    col = np.clip((C - B) * third + f(1.5), f(0), f(tr + 1))
+    return np.floor(row).astype(np.int64), np.floor(col).astype(np.int64)
 
 
-def test_hex_cell_floor_is_cube_rounding():
+def test_hex_cell_staged_is_cube_rounding():
     """The kernels' nearest-hexagon formula against cube rounding (the
     oracle's axial_round, hexagonal.py:32-39) on random axial points: the
-    same cell for every point farther than 1e-4 from a cell boundary (every
-    boundary lies on an integer line of 2q + r, -q - 2r or r - q), and a
-    planted off-by-one in the division is caught."""
+    same cell (row - 1, column - 1) for every point farther than 1e-4 from a
+    cell boundary (every boundary lies on an integer line of 2q + r, -q - 2r
+    or r - q); on a small table, exactly the cells off it land on the staged
+    border (row or column 0 or tq + 1 / tr + 1); and a planted off-by-one in
+    the division is caught."""
     rng = np.random.default_rng(3)
     q = rng.uniform(-300.0, 300.0, 400_000)
     r = rng.uniform(-300.0, 300.0, 400_000)
-    qi, ri = O.axial_round(q, r)
+    qi, ri = (np.asarray(a) for a in O.axial_round(q, r))
     # distance (in the pairwise differences) from the nearest integer line
     d = np.stack([2 * q + r, -q - 2 * r, r - q])
     margin = np.abs(d - np.round(d)).min(0)
     clear = margin > 1e-4
-    fq, fr = _hex_cell_floor(q, r)
+    # (a table of 1000 x 1000 cells holding the points shifted by 400 cells)
+    big = 1000
+    fq, fr = _hex_cell_staged(q + 400.0, r + 400.0, big, big)
     assert clear.mean() > 0.99
-    assert np.array_equal(fq[clear], np.asarray(qi)[clear])
-    assert np.array_equal(fr[clear], np.asarray(ri)[clear])
+    assert np.array_equal(fq[clear] - 401, qi[clear])
+    assert np.array_equal(fr[clear] - 401, ri[clear])
+    # a 40 x 25 table: on it, the cell; off it, a border row or column
+    tq, tr = 40, 25
+    sq, sr = _hex_cell_staged(q, r, tq, tr)
+    on = (qi >= 0) & (qi < tq) & (ri >= 0) & (ri < tr)
+    border = (sq == 0) | (sq == tq + 1) | (sr == 0) | (sr == tr + 1)
+    assert on[clear].any() and (~on[clear]).any()
+    assert np.array_equal(border[clear], ~on[clear])
+    assert np.array_equal(sq[clear & on] - 1, qi[clear & on])
+    assert np.array_equal(sr[clear & on] - 1, ri[clear & on])
+    assert sq.min() >= 0 and sq.max() <= tq + 1 and sr.min() >= 0 and sr.max() <= tr + 1
     # integer centres map to themselves, also at the table's far corner
-    c = np.arange(-5, 300, dtype=np.float64)
-    cq, cr = _hex_cell_floor(c, c[::-1].copy())
-    assert np.array_equal(cq, c.astype(np.int64)) and np.array_equal(cr, c[::-1].astype(np.int64))
+    c = np.arange(0, 1000, dtype=np.float64)
+    cq, cr = _hex_cell_staged(c, c[::-1].copy(), big, big)
+    assert np.array_equal(cq - 1, c.astype(np.int64))
+    assert np.array_equal(cr - 1, c[::-1].astype(np.int64))
     # planted mistake: (A - C) div 3 without the +1 misses the cells
     A = np.floor(2 * q + r)
     C = np.floor(r - q)
     bad = np.floor((A - C) / 3.0).astype(np.int64)
-    assert not np.array_equal(bad[clear], np.asarray(qi)[clear])
+    assert not np.array_equal(bad[clear], qi[clear])
