@@ -180,9 +180,8 @@ def c5(ia, configs, n_instances, steps, warmup, want_parity, want_plain):
             "bound": "valu", "achieved": bwd_tf, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": bwd_tf * 1e12 / PEAK_FP32,
             "traffic": profile_value("k_adj_sm_C5_counters.txt", "hbm_bytes_per_launch"),
-            "kernel": "k_adj_sm (source-major adjoint; PART 1 and PART 2 launches per "
-                      "instance), HIP events around iact_trace_batched_bwd on the launch "
-                      "stream, mean of 3",
+            "kernel": "k_adj_sm (source-major adjoint, one launch per instance), HIP "
+                      "events around iact_trace_batched_bwd on the launch stream, mean of 3",
             "kernel_ms": b_ms,
             "rocprof_kernel_ms": profile_value("k_adj_sm_C5_counters.txt",
                                                "kernel_ms_per_backward"),

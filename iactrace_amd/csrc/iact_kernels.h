@@ -18,17 +18,9 @@ hipError_t launch_fill_d(double* p, long long n, double v, hipStream_t st);
 hipError_t launch_sample(const iact_sample_desc& D, float* p, float* n, float* d, float* w,
                          hipStream_t st);
 hipError_t launch_world(const iact_facets& F, const DevSensor& S, float* out, hipStream_t st);
-// The trace kernels, by role.
-// TRACE_SOURCE_MAJOR: the source-major image trace (in obstructed scenes its
-// facet runs with obstruction candidates); TRACE_SOURCE_MAJOR_FREE: the runs
-// without candidates of an obstructed scene.
-enum TraceRole {
-  TRACE_IMAGE = 0,
-  TRACE_DEBUG = 1,
-  TRACE_RESPONSE = 2,
-  TRACE_SOURCE_MAJOR = 3,
-  TRACE_SOURCE_MAJOR_FREE = 4
-};
+// The trace kernels, by role (TRACE_SOURCE_MAJOR: the source-major image
+// trace).
+enum TraceRole { TRACE_IMAGE = 0, TRACE_DEBUG = 1, TRACE_RESPONSE = 2, TRACE_SOURCE_MAJOR = 3 };
 using TraceKernel = void (*)(TraceArgs);
 TraceKernel pick_trace_kernel(const TraceArgs& A, int source_type, int which);
 hipError_t launch_trace_kernel(TraceKernel k, const TraceArgs& A, const LaunchShape& L,

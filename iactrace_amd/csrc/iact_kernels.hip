@@ -735,18 +735,19 @@ IACT_DEV bool refine_candidates(const TraceArgs& A, int f, int c, int blk, int l
 // pixel (the samples of one facet put a source's rays on one or two pixels)
 // and adds the run to the LDS image when the pixel changes; splat taps are
 // added one by one.  The per-ray arithmetic is trace_ray's, as in the
-// sample-major kernel.  In obstructed scenes (OBST) the work splits by the
-// facet runs of the items: PART 1 traces the runs without an obstruction
-// candidate for any of the lane's chunks (most of them when culling), PART 2
-// those with one, with shadow tests; the host launches both over the same
-// items, so each kernel's registers are allocated for its own loop.
-// PART 2 is held to 72 VGPRs (7 waves per SIMD): its shadow tests wait on
-// table loads, and the extra waves hide more of that than the few spills
-// (outside the per-ray loops) cost -- measured 10 % faster on the alignment
-// fit's instances than at its natural 84 VGPRs (5 waves); 8 waves lose 40 %.
-template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL, int PART>
+// sample-major kernel.  In obstructed scenes (OBST) each facet run of an
+// item takes one of two paths, uniform over the wave: runs without an
+// obstruction candidate for any of the lane's chunks (most of them when
+// culling) are traced without shadow tests, runs with one take the shadow
+// path (refined candidates, far_run_shadow bits or per-ray tests).  The
+// kernel is held to 80 VGPRs (6 waves per SIMD; the shadow path alone would
+// take 84): on the alignment fit's instances one such launch is 7.6 % faster
+// than the two launches of an earlier split by path (each over every item,
+// each with its own register allocation), and 7 % faster than at 5 or 7
+// waves, whose register limits move spills into the per-ray loop.
+template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL>
 __global__ __launch_bounds__(IACT_MAX_BLOCK)
-__attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArgs A) {
+__attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) {
   extern __shared__ float smem[];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
@@ -892,16 +893,12 @@ __attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArg
         any |= fc[k].any;
       }
       const bool shadow = OBST && any;
-      if (OBST && (PART == 1 ? shadow : !shadow)) {  // the other kernel's run
-        j = jend;
-        continue;
-      }
-      // the shadow kernel re-tests the candidates of a run inside one block
+      // the shadow path re-tests the candidates of a run inside one block
       // of 64 samples against the block's bundle (the block's ball, the
       // chunk's cone), one candidate per lane; lists of more than 64 are not
       // refined
       refined = false;
-      if (OBST && PART == 2 && A.gbounds && (j >> 6) == ((jend - 1) >> 6))
+      if (shadow && A.gbounds && (j >> 6) == ((jend - 1) >> 6))
         refined = refine_candidates<SPL>(A, f, c, j >> 6, lane, fc, rlist, rbits);
       const cptr<float> fp = as_const(A.positions) + 3 * f;  // scalar loads
       const f3 pos = mk3(fp[0], fp[1], fp[2]);
@@ -936,7 +933,7 @@ __attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArg
       };
       using T_ = std::true_type;
       using F_ = std::false_type;
-      if (OBST && PART == 2) {
+      if (shadow) {
         // (no generic copy: the shadow tests dominate, and more copies would
         // cost this kernel scalar registers)
         if (all_far && refined) {
@@ -968,6 +965,7 @@ __attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArg
       } else {
         run(F_{}, T_{}, F_{}, F_{});
       }
+      j = jend;  // (run has traced up to the run's end)
     }
     if (!SPLAT) {
 #pragma unroll
@@ -988,7 +986,7 @@ __attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArg
 
 // Adjoint of the source-major image trace for primary-only, hard-binned
 // renders (the reference gets it from jax.grad through render.py:174-220).
-// Lanes, items, records, obstruction runs (PART 1 / PART 2) and the per-ray
+// Lanes, items, records, the two paths of obstructed scenes and the per-ray
 // arithmetic up to the pixel are k_trace_sm's, so every ray lands on the
 // pixel the forward gave it; the pixel's image gradient G is read from LDS.
 // With hard binning only the ray value v = value (-d.n) / w * shadow
@@ -1006,10 +1004,11 @@ __attribute__((amdgpu_waves_per_eu(PART == 2 ? 7 : 1))) void k_trace_sm(TraceArg
 // fp64 per-facet sums go to k_torque_to_euler, which dots them with the axes.
 // Source and value gradients are summed per lane (per source) over an item's
 // samples.
-// Held to 80 VGPRs (6 waves per SIMD), for the reason PART 2 of k_trace_sm
-// is: measured 3 % (PART 1, natural 82 VGPRs) and 9 % (PART 2, natural 112)
-// faster on the alignment fit's instances; 7 waves for PART 2 lose 8 %.
-template <int SRC, int KIND, bool AXIS, int OBST, int SPL, int PART>
+// Held to 80 VGPRs (6 waves per SIMD; the shadow path alone would take 112):
+// on the alignment fit's instances 14 % faster than the two launches of an
+// earlier split by path at their natural allocations, 1 % faster than at 5
+// waves.
+template <int SRC, int KIND, bool AXIS, int OBST, int SPL>
 __global__ __launch_bounds__(IACT_MAX_BLOCK)
 __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
   const TraceArgs& A = B.T;
@@ -1188,12 +1187,8 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
         any |= fc[k].any;
       }
       const bool shadow = OBST && any;
-      if (OBST && (PART == 1 ? shadow : !shadow)) {
-        j = jend;
-        continue;
-      }
       refined = false;
-      if (OBST && PART == 2 && A.gbounds && (j >> 6) == ((jend - 1) >> 6))
+      if (shadow && A.gbounds && (j >> 6) == ((jend - 1) >> 6))
         refined = refine_candidates<SPL>(A, f, c, j >> 6, lane, fc, rlist, rbits);
       const cptr<float> fp = as_const(A.positions) + 3 * f;
       const f3 pos = mk3(fp[0], fp[1], fp[2]);
@@ -1235,7 +1230,7 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
       };
       using T_ = std::true_type;
       using F_ = std::false_type;
-      if (OBST && PART == 2) {
+      if (shadow) {
         if (all_far && refined) {
           // the forward's shadow bits of the run (far_run_shadow), then the
           // run without shadow tests
@@ -1264,6 +1259,7 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
       } else {
         run(F_{}, T_{}, F_{}, F_{});
       }
+      j = jend;  // (run has traced up to the run's end)
       if (all_far) {
 #pragma unroll
         for (int k = 0; k < SPL; ++k) {
@@ -1438,25 +1434,19 @@ static TraceKernel pick_trace_t(const TraceArgs& A) {
 }
 
 // source-major kernels exist for primary-only scenes (the host selects that
-// layout only for those); obstructed scenes take two kernels, for the runs
-// with obstruction candidates (TRACE_SOURCE_MAJOR) and those without
-// (TRACE_SOURCE_MAJOR_FREE)
+// layout only for those); obstructed scenes take the kernel with the shadow
+// path
 template <int SRC, int KIND, int SPLAT, int SPL>
-static TraceKernel pick_sm_spl(const TraceArgs& A, int which) {
-  if (has_obstructions(A.ob)) {
-    if (which == TRACE_SOURCE_MAJOR_FREE)
-      return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL, 1>
-                      : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL, 1>;
-    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL, 2>
-                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL, 2>;
-  }
-  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0, SPL, 0>
-                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0, SPL, 0>;
+static TraceKernel pick_sm_spl(const TraceArgs& A) {
+  if (has_obstructions(A.ob))
+    return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 1, SPL>
+                    : k_trace_sm<SRC, KIND, SPLAT, false, false, 1, SPL>;
+  return A.S.axis ? k_trace_sm<SRC, KIND, SPLAT, false, true, 0, SPL>
+                  : k_trace_sm<SRC, KIND, SPLAT, false, false, 0, SPL>;
 }
 template <int SRC, int KIND, int SPLAT>
-static TraceKernel pick_sm_t(const TraceArgs& A, int which) {
-  return A.sm_spl == 2 ? pick_sm_spl<SRC, KIND, SPLAT, 2>(A, which)
-                       : pick_sm_spl<SRC, KIND, SPLAT, 1>(A, which);
+static TraceKernel pick_sm_t(const TraceArgs& A) {
+  return A.sm_spl == 2 ? pick_sm_spl<SRC, KIND, SPLAT, 2>(A) : pick_sm_spl<SRC, KIND, SPLAT, 1>(A);
 }
 
 template <int SRC>
@@ -1473,11 +1463,9 @@ static TraceKernel pick_src(const TraceArgs& A, int which) {
       return sq ? pick_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_DEBUG>(A)
                 : pick_trace_t<SRC, IACT_SENSOR_HEX, 0, IACT_MODE_DEBUG>(A);
     case TRACE_SOURCE_MAJOR:
-    case TRACE_SOURCE_MAJOR_FREE:
-      if (sq) return sp ? pick_sm_t<SRC, IACT_SENSOR_SQUARE, 1>(A, which)
-                        : pick_sm_t<SRC, IACT_SENSOR_SQUARE, 0>(A, which);
-      return sp ? pick_sm_t<SRC, IACT_SENSOR_HEX, 1>(A, which)
-                : pick_sm_t<SRC, IACT_SENSOR_HEX, 0>(A, which);
+      if (sq) return sp ? pick_sm_t<SRC, IACT_SENSOR_SQUARE, 1>(A)
+                        : pick_sm_t<SRC, IACT_SENSOR_SQUARE, 0>(A);
+      return sp ? pick_sm_t<SRC, IACT_SENSOR_HEX, 1>(A) : pick_sm_t<SRC, IACT_SENSOR_HEX, 0>(A);
     default:
       if (sq) return sp ? pick_trace_t<SRC, IACT_SENSOR_SQUARE, 1, IACT_MODE_IMAGE>(A)
                         : pick_trace_t<SRC, IACT_SENSOR_SQUARE, 0, IACT_MODE_IMAGE>(A);
@@ -1510,37 +1498,25 @@ hipError_t launch_trace(const TraceArgs& A, const LaunchShape& L, int source_typ
 
 hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_type,
                            hipStream_t st) {
-  // obstructed scenes: the runs without candidates first (none when every
-  // mask bit is set), then those with
-  if (has_obstructions(A.ob) && !A.cull_all) {
-    const hipError_t e =
-        launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR_FREE), A, L, st);
-    if (e != hipSuccess) return e;
-  }
   return launch_trace_kernel(pick_trace_kernel(A, source_type, TRACE_SOURCE_MAJOR), A, L, st);
 }
 
 template <int SRC, int KIND, bool AXIS, int SPL>
-static const void* pick_adj_sm_spl(const AdjArgs& B, int which) {
-  if (has_obstructions(B.T.ob)) {
-    if (which == TRACE_SOURCE_MAJOR_FREE) return (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL, 1>;
-    return (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL, 2>;
-  }
-  return (const void*)k_adj_sm<SRC, KIND, AXIS, 0, SPL, 0>;
+static const void* pick_adj_sm_spl(const AdjArgs& B) {
+  if (has_obstructions(B.T.ob)) return (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL>;
+  return (const void*)k_adj_sm<SRC, KIND, AXIS, 0, SPL>;
 }
 template <int SRC, int KIND>
-static const void* pick_adj_sm_k(const AdjArgs& B, int which) {
+static const void* pick_adj_sm_k(const AdjArgs& B) {
   const bool ax = B.T.S.axis != 0;
   if (B.T.sm_spl == 2)
-    return ax ? pick_adj_sm_spl<SRC, KIND, true, 2>(B, which)
-              : pick_adj_sm_spl<SRC, KIND, false, 2>(B, which);
-  return ax ? pick_adj_sm_spl<SRC, KIND, true, 1>(B, which)
-            : pick_adj_sm_spl<SRC, KIND, false, 1>(B, which);
+    return ax ? pick_adj_sm_spl<SRC, KIND, true, 2>(B) : pick_adj_sm_spl<SRC, KIND, false, 2>(B);
+  return ax ? pick_adj_sm_spl<SRC, KIND, true, 1>(B) : pick_adj_sm_spl<SRC, KIND, false, 1>(B);
 }
 template <int SRC>
-static const void* pick_adj_sm(const AdjArgs& B, int which) {
-  return B.T.S.kind == IACT_SENSOR_SQUARE ? pick_adj_sm_k<SRC, IACT_SENSOR_SQUARE>(B, which)
-                                          : pick_adj_sm_k<SRC, IACT_SENSOR_HEX>(B, which);
+static const void* pick_adj_sm(const AdjArgs& B) {
+  return B.T.S.kind == IACT_SENSOR_SQUARE ? pick_adj_sm_k<SRC, IACT_SENSOR_SQUARE>(B)
+                                          : pick_adj_sm_k<SRC, IACT_SENSOR_HEX>(B);
 }
 
 hipError_t launch_adjoint_sm(const AdjArgs& B, const LaunchShape& L, int source_type,
@@ -1548,18 +1524,11 @@ hipError_t launch_adjoint_sm(const AdjArgs& B, const LaunchShape& L, int source_
   if (has_obstructions(B.T.ob) && !B.T.cull) return hipErrorInvalidValue;
   if (B.T.S.splat || B.T.n_stages > 0 || B.T.layout != 2 || !B.T.recs || !B.torque)
     return hipErrorInvalidValue;
-  auto go = [&](int which) {
-    AdjArgs b = B;
-    void* args[] = {&b};
-    const void* k = source_type == IACT_SOURCE_POINT ? pick_adj_sm<IACT_SOURCE_POINT>(B, which)
-                                                     : pick_adj_sm<IACT_SOURCE_PARALLEL>(B, which);
-    return hipLaunchKernel(k, dim3(L.grid), dim3(L.threads), args, L.lds, st);
-  };
-  if (has_obstructions(B.T.ob) && !B.T.cull_all) {
-    const hipError_t e = go(TRACE_SOURCE_MAJOR_FREE);
-    if (e != hipSuccess) return e;
-  }
-  return go(TRACE_SOURCE_MAJOR);
+  AdjArgs b = B;
+  void* args[] = {&b};
+  const void* k = source_type == IACT_SOURCE_POINT ? pick_adj_sm<IACT_SOURCE_POINT>(B)
+                                                   : pick_adj_sm<IACT_SOURCE_PARALLEL>(B);
+  return hipLaunchKernel(k, dim3(L.grid), dim3(L.threads), args, L.lds, st);
 }
 
 hipError_t launch_torque_to_euler(const double* torque, const float* rotations, int nf,
