@@ -1008,7 +1008,9 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
 // on the alignment fit's instances 14 % faster than the two launches of an
 // earlier split by path at their natural allocations, 1 % faster than at 5
 // waves.
-template <int SRC, int KIND, bool AXIS, int OBST, int SPL>
+// GX false: the launch wants rotation gradients only (no value, source or
+// position gradients), and the kernel has no code or registers for the rest.
+template <int SRC, int KIND, bool AXIS, int OBST, int SPL, bool GX>
 __global__ __launch_bounds__(IACT_MAX_BLOCK)
 __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
   const TraceArgs& A = B.T;
@@ -1027,8 +1029,8 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
   const long long total = (long long)A.nf * A.M;
   const bool lean = as_const((const uint32_t*)(A.recs + IACT_REC_WORDS * total))[0] == 0u &&
                     !A.S.edge && (KIND != IACT_SENSOR_SQUARE || A.S.v_scale == 1.f);
-  const bool want_src = B.g_src != nullptr, want_val = B.g_val != nullptr;
-  const bool want_pos = B.g_pos != nullptr;
+  const bool want_src = GX && B.g_src != nullptr, want_val = GX && B.g_val != nullptr;
+  const bool want_pos = GX && B.g_pos != nullptr;
   for (long long item = xcd_block(A.xcd_order) * wpb + wave; item < A.sm_items;
        item += (long long)grid_blocks() * wpb) {
     const uint32_t g = fdiv((uint32_t)item, A.d_sc);
@@ -1074,16 +1076,13 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
     const bool want_pc = want_pos || want_src;
     // one sample's rays (SPL per lane): the forward's trace up to the pixel,
     // then the lane's share of the sample's torque and dL/dp (tq, gp), or
-    // (FR) the far sums.  GX false: the launch wants no value, source or
-    // position gradients (rotations only), and the loop has no code for them.
-    auto adjoint_one = [&](auto sh, auto gen, auto far, auto sbits, auto gx, const RecRaw& rec,
-                           int bi, f3& tq, f3& gp) {
+    // (FR) the far sums.
+    auto adjoint_one = [&](auto sh, auto gen, auto far, auto sbits, const RecRaw& rec, int bi,
+                           f3& tq, f3& gp) {
       constexpr int OB = decltype(sh)::value ? OBST : 0;
       constexpr bool GEN = decltype(gen)::value;
       constexpr bool FR = decltype(far)::value;
       constexpr bool SB = decltype(sbits)::value;
-      constexpr bool GX = decltype(gx)::value;
-      const bool wval = GX && want_val, wsrc = GX && want_src, wpc = GX && want_pc;
       const Sample smp0 = unpack_record(rec);
       float san_x = 0.f, san_y = 0.f, san_z = 0.f, st = 0.f;
       float sw_x = 0.f, sw_y = 0.f, sw_z = 0.f;
@@ -1125,7 +1124,7 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
         // a = G value shadow; dc = a nrw is applied per sample
         const float gs = (OB || SB) ? gp_pix * shadow : gp_pix;
         const float a = gs * value[k];
-        if (wval) {
+        if (want_val) {
           // dL/dvalue = G v / value = G shadow (d.m) nrwh
           gval[k] = fmaf(gs * (c2u * inv), smp.nrwh, gval[k]);
         }
@@ -1133,7 +1132,7 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
           // b n = bm m; the source's value scales the run's sums once
           const float bm = gs * smp.nrwh;
           fN[k] = fma3(bm, smp.m, fN[k]);
-          if (wpc) fC[k] = fmaf(bm, c2u * inv, fC[k]);
+          if (want_pc) fC[k] = fmaf(bm, c2u * inv, fC[k]);
         } else if (SRC == IACT_SOURCE_POINT) {
           const float k1 = a * inv;  // dc / |u| before nrw
           san_x = fmaf(k1, u.x, san_x);
@@ -1145,7 +1144,7 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
           sw_x = fmaf(k2, u.x, sw_x);
           sw_y = fmaf(k2, u.y, sw_y);
           sw_z = fmaf(k2, u.z, sw_z);
-          if (wsrc) {
+          if (want_src) {
             // dL/dsrc = -dL/du = -(dc / |u|) (n - (d.n) d)
             const float tsrc = k1 * smp.nrw;
             const float cdn = -0.5f * c2u * inv;
@@ -1156,7 +1155,7 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
           san_x = fmaf(a, d.x, san_x);
           san_y = fmaf(a, d.y, san_y);
           san_z = fmaf(a, d.z, san_z);
-          if (wsrc) {
+          if (want_src) {
             // d = the source vector as given: dL/dsrc = dc n
             gsrc[k] = fma3(a * smp.nrw, smp.n, gsrc[k]);
           }
@@ -1209,24 +1208,18 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
       RecRaw cur = fetch_record(rp);
       f3 T = mk3(0.f, 0.f, 0.f), Gp = mk3(0.f, 0.f, 0.f);
       const int jb = j;
-      auto run_gx = [&](auto sh, auto gen, auto far, auto sbits, auto gx) {
+      auto run = [&](auto sh, auto gen, auto far, auto sbits) {
         for (; j < jend; ++j) {
           rp += 4;
           const RecRaw nxt = fetch_record(rp);
           f3 tq, gp;
-          adjoint_one(sh, gen, far, sbits, gx, cur, j - jb, tq, gp);
+          adjoint_one(sh, gen, far, sbits, cur, j - jb, tq, gp);
           if (!decltype(far)::value) {
             T = T + tq;
             Gp = Gp + gp;
           }
           cur = nxt;
         }
-      };
-      auto run = [&](auto sh, auto gen, auto far, auto sbits) {
-        if (want_val || want_pc)
-          run_gx(sh, gen, far, sbits, std::true_type{});
-        else
-          run_gx(sh, gen, far, sbits, std::false_type{});
       };
       using T_ = std::true_type;
       using F_ = std::false_type;
@@ -1503,8 +1496,12 @@ hipError_t launch_trace_sm(const TraceArgs& A, const LaunchShape& L, int source_
 
 template <int SRC, int KIND, bool AXIS, int SPL>
 static const void* pick_adj_sm_spl(const AdjArgs& B) {
-  if (has_obstructions(B.T.ob)) return (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL>;
-  return (const void*)k_adj_sm<SRC, KIND, AXIS, 0, SPL>;
+  const bool gx = B.g_val || B.g_src || B.g_pos;
+  if (has_obstructions(B.T.ob))
+    return gx ? (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL, true>
+              : (const void*)k_adj_sm<SRC, KIND, AXIS, 1, SPL, false>;
+  return gx ? (const void*)k_adj_sm<SRC, KIND, AXIS, 0, SPL, true>
+            : (const void*)k_adj_sm<SRC, KIND, AXIS, 0, SPL, false>;
 }
 template <int SRC, int KIND>
 static const void* pick_adj_sm_k(const AdjArgs& B) {
