@@ -934,8 +934,9 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
       using T_ = std::true_type;
       using F_ = std::false_type;
       if (shadow) {
-        // (no generic copy: the shadow tests dominate, and more copies would
-        // cost this kernel scalar registers)
+        // (the runs with per-ray shadow tests have no lean copy: their tests
+        // dominate; the shadow-bit runs do, since they trace like the
+        // shadow-free runs)
         if (all_far && refined) {
           // far sources over a refined run: the candidates' shadow bits
           // first, then the run without shadow tests
@@ -951,7 +952,10 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
 #pragma unroll
           for (int k = 0; k < SPL; ++k)
             blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k]);
-          run(F_{}, T_{}, T_{}, T_{});
+          if (lean)
+            run(F_{}, F_{}, T_{}, T_{});
+          else
+            run(F_{}, T_{}, T_{}, T_{});
         } else if (all_far) {
           run(T_{}, T_{}, T_{}, F_{});
         } else {
@@ -1238,7 +1242,10 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
             for (int k = 0; k < SPL; ++k)
               blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k]);
           }
-          run(F_{}, T_{}, T_{}, T_{});
+          if (lean)
+            run(F_{}, F_{}, T_{}, T_{});
+          else
+            run(F_{}, T_{}, T_{}, T_{});
         } else if (all_far) {
           run(T_{}, T_{}, T_{}, F_{});
         } else {
