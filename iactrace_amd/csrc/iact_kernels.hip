@@ -230,94 +230,6 @@ __global__ __launch_bounds__(256) void k_facet_cones(TraceArgs A, int source_typ
   }
 }
 
-// Can a ray of the bundle (centre c, radius rho, axis a, tan theta) pass
-// within R of b?  Along the axis the ray point lies in [-rho + t cos, rho + t],
-// across it within rho + t sin of the axis, so it is necessary that
-// s + rho + R >= 0 and e <= rho + R + tan(theta) (s + rho + R), with s, e the
-// axial and radial coordinates of b.  Radii get 0.1% + 1 mm of slack.
-IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
-  const f3 w = b - c;
-  const float s = dot(w, a);
-  const f3 perp = fma3(-s, a, w);
-  const float e = sqrtf(dot(perp, perp));
-  const float rr = fmaf(R, 1.001f, 1e-3f) + rho;
-  // written so that NaN geometry keeps the candidate
-  return !(s + rr < 0.f) && !(e > fmaf(tn, s + rr, rr));
-}
-
-// Can a ray of the bundle reach obstruction j (mask numbering: cylinders,
-// boxes, spheres, oriented boxes, triangles)?  Cylinders are tested as the
-// capsule of their axis segment (cylinder_reaches), the other kinds through
-// a bounding sphere.
-IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn);
-IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
-                                  float tn) {
-  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
-  const int c3 = c2 + ob.n_oriented_boxes;
-  if (j < c0) return cylinder_reaches(ob.cylinders + 8 * j, c, rho, a, tn);
-  if (j < c1) {
-    const float* q = ob.boxes + 6 * (j - c0);
-    const f3 lo = mk3(q[0], q[1], q[2]), hi = mk3(q[3], q[4], q[5]);
-    const f3 dd = hi - lo;
-    return bundle_reaches(c, rho, a, tn, 0.5f * (lo + hi), 0.5f * sqrtf(dot(dd, dd)));
-  }
-  if (j < c2) {
-    const float* q = ob.spheres + 4 * (j - c1);
-    return bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), q[3]);
-  }
-  if (j < c3) {
-    const float* q = ob.oriented_boxes + 15 * (j - c2);
-    const f3 he = mk3(q[3], q[4], q[5]);
-    return bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), sqrtf(dot(he, he)));
-  }
-  const float* q = ob.triangles + 9 * (j - c3);
-  const f3 v0 = mk3(q[0], q[1], q[2]), v1 = mk3(q[3], q[4], q[5]);
-  const f3 v2 = mk3(q[6], q[7], q[8]);
-  const f3 ctr = (1.f / 3.f) * (v0 + v1 + v2);
-  const f3 e0 = v0 - ctr, e1 = v1 - ctr, e2 = v2 - ctr;
-  const float R = sqrtf(fmaxf(fmaxf(dot(e0, e0), dot(e1, e1)), dot(e2, e2)));
-  return bundle_reaches(c, rho, a, tn, ctr, R);
-}
-
-// Can a ray of the bundle reach cylinder q (p1, unit axis, height, radius)?
-// The cylinder lies in the capsule of its axis segment, so it is necessary
-// that bundle_reaches holds for some axis point b(t) = p1 + t axis, t in
-// [0, h], with R = the radius: along the segment the axial coordinate s(t)
-// and the bound g(t) = rr + tan(theta) (s(t) + rr) are linear and the radial
-// distance squared e(t)^2 quadratic, so the test is whether
-// Q(t) = e(t)^2 - g(t)^2 <= 0 somewhere on the interval where s(t) >= -rr
-// and g(t) >= 0 (ends and the vertex of Q).  Q gets a tolerance of 1e-5 of
-// its terms' magnitude; NaN geometry keeps the candidate.
-IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn) {
-  const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
-  const float h = q[6], r = q[7];
-  const float rr = fmaf(r, 1.001f, 1e-3f) + rho;
-  const f3 w0 = p1 - c;
-  const float s0 = dot(w0, a), s1 = dot(ax, a);
-  const f3 P0 = fma3(-s0, a, w0), P1 = fma3(-s1, a, ax);
-  const float G0 = fmaf(tn, s0 + rr, rr), G1 = tn * s1;
-  // interval of t with s(t) + rr >= 0 and g(t) >= 0
-  float lo = 0.f, hi = h;
-  auto clip = [&](float k0, float k1) {  // k0 + k1 t >= 0
-    if (k1 > 0.f) lo = fmaxf(lo, -k0 / k1);
-    else if (k1 < 0.f) hi = fminf(hi, -k0 / k1);
-    else if (k0 < 0.f) hi = -1.f;
-  };
-  clip(s0 + rr, s1);
-  clip(G0, G1);
-  const float qa = dot(P1, P1) - G1 * G1;
-  const float qb = 2.f * (dot(P0, P1) - G0 * G1);
-  const float qc = dot(P0, P0) - G0 * G0;
-  auto Q = [&](float t) { return fmaf(fmaf(qa, t, qb), t, qc); };
-  float m = fminf(Q(lo), Q(hi));
-  if (qa > 0.f) m = fminf(m, Q(fminf(fmaxf(-qb / (2.f * qa), lo), hi)));
-  const float hh = fmaxf(fabsf(lo), fabsf(hi));
-  const float tol =
-      1e-5f * (dot(P0, P0) + G0 * G0 + hh * hh * (dot(P1, P1) + G1 * G1) + 1e-6f);
-  if (!(lo <= hi)) return lo != lo || hi != hi;  // empty interval (NaN: keep)
-  return !(m > tol);
-}
-
 // Candidate bits: one thread per (facet, mask word).
 __global__ void k_cull_masks(TraceArgs A, const float* cones, uint32_t* masks, int all) {
   const iact_obstructions& ob = A.ob;
@@ -949,9 +861,11 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
           // (one chunk at a time: the two-direction walk of the adjoint
           // costs this kernel two waves per SIMD of occupancy)
           const float* rec0 = (const float*)(recs + 4 * (size_t)jb);
+          const cptr<float> gb = as_const(A.gbounds) + 4 * (size_t)(jb >> 6);  // the run's ball
+          const f3 gc = mk3(gb[0], gb[1], gb[2]);
 #pragma unroll
           for (int k = 0; k < SPL; ++k)
-            blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k]);
+            blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k], gc, gb[3]);
           if (lean)
             run(F_{}, F_{}, T_{}, T_{});
           else
@@ -1235,12 +1149,14 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
 #pragma unroll
           for (int k = 0; k < SPL; ++k) dsh[k] = -fd[k];
           const float* rec0 = (const float*)(recs + 4 * (size_t)jb);
+          const cptr<float> gb = as_const(A.gbounds) + 4 * (size_t)(jb >> 6);  // the run's ball
+          const f3 gc = mk3(gb[0], gb[1], gb[2]);
           if constexpr (SPL == 2) {
-            far_run_shadow2(A, rec0, jend - jb, rlist, rbits, dsh, blk);
+            far_run_shadow2(A, rec0, jend - jb, rlist, rbits, dsh, gc, gb[3], blk);
           } else {
 #pragma unroll
             for (int k = 0; k < SPL; ++k)
-              blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k]);
+              blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k], gc, gb[3]);
           }
           if (lean)
             run(F_{}, F_{}, T_{}, T_{});

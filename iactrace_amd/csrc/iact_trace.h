@@ -556,6 +556,94 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
                               s.cand_any, s.p, -d, active);
 }
 
+// Can a ray of the bundle (centre c, radius rho, axis a, tan theta) pass
+// within R of b?  Along the axis the ray point lies in [-rho + t cos, rho + t],
+// across it within rho + t sin of the axis, so it is necessary that
+// s + rho + R >= 0 and e <= rho + R + tan(theta) (s + rho + R), with s, e the
+// axial and radial coordinates of b.  Radii get 0.1% + 1 mm of slack.
+IACT_DEV bool bundle_reaches(f3 c, float rho, f3 a, float tn, f3 b, float R) {
+  const f3 w = b - c;
+  const float s = dot(w, a);
+  const f3 perp = fma3(-s, a, w);
+  const float e = sqrtf(dot(perp, perp));
+  const float rr = fmaf(R, 1.001f, 1e-3f) + rho;
+  // written so that NaN geometry keeps the candidate
+  return !(s + rr < 0.f) && !(e > fmaf(tn, s + rr, rr));
+}
+
+// Can a ray of the bundle reach obstruction j (mask numbering: cylinders,
+// boxes, spheres, oriented boxes, triangles)?  Cylinders are tested as the
+// capsule of their axis segment (cylinder_reaches), the other kinds through
+// a bounding sphere.
+IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn);
+IACT_DEV bool obstruction_reaches(const iact_obstructions& ob, int j, f3 c, float rho, f3 a,
+                                  float tn) {
+  const int c0 = ob.n_cylinders, c1 = c0 + ob.n_boxes, c2 = c1 + ob.n_spheres;
+  const int c3 = c2 + ob.n_oriented_boxes;
+  if (j < c0) return cylinder_reaches(ob.cylinders + 8 * j, c, rho, a, tn);
+  if (j < c1) {
+    const float* q = ob.boxes + 6 * (j - c0);
+    const f3 lo = mk3(q[0], q[1], q[2]), hi = mk3(q[3], q[4], q[5]);
+    const f3 dd = hi - lo;
+    return bundle_reaches(c, rho, a, tn, 0.5f * (lo + hi), 0.5f * sqrtf(dot(dd, dd)));
+  }
+  if (j < c2) {
+    const float* q = ob.spheres + 4 * (j - c1);
+    return bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), q[3]);
+  }
+  if (j < c3) {
+    const float* q = ob.oriented_boxes + 15 * (j - c2);
+    const f3 he = mk3(q[3], q[4], q[5]);
+    return bundle_reaches(c, rho, a, tn, mk3(q[0], q[1], q[2]), sqrtf(dot(he, he)));
+  }
+  const float* q = ob.triangles + 9 * (j - c3);
+  const f3 v0 = mk3(q[0], q[1], q[2]), v1 = mk3(q[3], q[4], q[5]);
+  const f3 v2 = mk3(q[6], q[7], q[8]);
+  const f3 ctr = (1.f / 3.f) * (v0 + v1 + v2);
+  const f3 e0 = v0 - ctr, e1 = v1 - ctr, e2 = v2 - ctr;
+  const float R = sqrtf(fmaxf(fmaxf(dot(e0, e0), dot(e1, e1)), dot(e2, e2)));
+  return bundle_reaches(c, rho, a, tn, ctr, R);
+}
+
+// Can a ray of the bundle reach cylinder q (p1, unit axis, height, radius)?
+// The cylinder lies in the capsule of its axis segment, so it is necessary
+// that bundle_reaches holds for some axis point b(t) = p1 + t axis, t in
+// [0, h], with R = the radius: along the segment the axial coordinate s(t)
+// and the bound g(t) = rr + tan(theta) (s(t) + rr) are linear and the radial
+// distance squared e(t)^2 quadratic, so the test is whether
+// Q(t) = e(t)^2 - g(t)^2 <= 0 somewhere on the interval where s(t) >= -rr
+// and g(t) >= 0 (ends and the vertex of Q).  Q gets a tolerance of 1e-5 of
+// its terms' magnitude; NaN geometry keeps the candidate.
+IACT_DEV bool cylinder_reaches(const float* q, f3 c, float rho, f3 a, float tn) {
+  const f3 p1 = mk3(q[0], q[1], q[2]), ax = mk3(q[3], q[4], q[5]);
+  const float h = q[6], r = q[7];
+  const float rr = fmaf(r, 1.001f, 1e-3f) + rho;
+  const f3 w0 = p1 - c;
+  const float s0 = dot(w0, a), s1 = dot(ax, a);
+  const f3 P0 = fma3(-s0, a, w0), P1 = fma3(-s1, a, ax);
+  const float G0 = fmaf(tn, s0 + rr, rr), G1 = tn * s1;
+  // interval of t with s(t) + rr >= 0 and g(t) >= 0
+  float lo = 0.f, hi = h;
+  auto clip = [&](float k0, float k1) {  // k0 + k1 t >= 0
+    if (k1 > 0.f) lo = fmaxf(lo, -k0 / k1);
+    else if (k1 < 0.f) hi = fminf(hi, -k0 / k1);
+    else if (k0 < 0.f) hi = -1.f;
+  };
+  clip(s0 + rr, s1);
+  clip(G0, G1);
+  const float qa = dot(P1, P1) - G1 * G1;
+  const float qb = 2.f * (dot(P0, P1) - G0 * G1);
+  const float qc = dot(P0, P0) - G0 * G0;
+  auto Q = [&](float t) { return fmaf(fmaf(qa, t, qb), t, qc); };
+  float m = fminf(Q(lo), Q(hi));
+  if (qa > 0.f) m = fminf(m, Q(fminf(fmaxf(-qb / (2.f * qa), lo), hi)));
+  const float hh = fmaxf(fabsf(lo), fabsf(hi));
+  const float tol =
+      1e-5f * (dot(P0, P0) + G0 * G0 + hh * hh * (dot(P1, P1) + G1 * G1) + 1e-6f);
+  if (!(lo <= hi)) return lo != lo || hi != hi;  // empty interval (NaN: keep)
+  return !(m > tol);
+}
+
 // Shadow bits of a facet run of far sources (source_qs 0): bit i is set when
 // the shadow ray of the run's sample i towards the lane's source -- the
 // direction dsh = -d, fixed over the run -- meets one of the candidates that
@@ -565,9 +653,14 @@ IACT_DEV float primary_shadow(const TraceArgs& A, const Sample& s, f3 d, bool ac
 // per-ray kernels' own expressions (hit_cylinder = hit_cylinder_dir of
 // cyl_dir, and so on), so every ray is decided as they decide it.  The run's
 // n <= 64 sample points come one per lane (vector loads of the records' p
-// words) and are handed out with v_readlane.
+// words) and are handed out with v_readlane.  Before its samples a
+// candidate is tested per lane with the lane's own direction against the
+// ball (gc, grho) that holds the run's samples (obstruction_reaches, the
+// culling test with a bundle of zero angle): a lane it cannot reach counts
+// as done for it, and a candidate that reaches no lane is skipped -- a ray
+// it cannot reach it does not block, so the bits are unchanged.
 IACT_DEV uint64_t far_run_shadow(const TraceArgs& A, const float* rec0, int n, int list,
-                                 uint64_t bits, f3 dsh) {
+                                 uint64_t bits, f3 dsh, f3 gc, float grho) {
   const iact_obstructions& ob = A.ob;
   const int lane = lane_id();
   const float* pr = rec0 + (size_t)IACT_REC_WORDS * (lane < n ? lane : 0) + 13;
@@ -578,23 +671,25 @@ IACT_DEV uint64_t far_run_shadow(const TraceArgs& A, const float* rec0, int n, i
     const int pos = __builtin_ctzll(bits);
     bits &= bits - 1ull;
     const int j = __builtin_amdgcn_readlane(list, pos);
+    const bool out = !obstruction_reaches(ob, j, gc, grho, dsh, 0.f);
+    if (__all(out)) continue;
     if (j < c0) {
       const cptr<float> c = as_const(ob.cylinders) + 8 * j;
       const CylDir D = cyl_dir(dsh, c);
       for (int i = 0; i < n; ++i) {
-        if (__all((blocked >> i) & 1ull)) continue;
+        if (__all(out || ((blocked >> i) & 1ull))) continue;
         if (hit_cylinder_dir(lane_value(myp, i), D, c) < 1e10f) blocked |= 1ull << i;
       }
     } else if (j < c1) {
       const cptr<float> b = as_const(ob.boxes) + 6 * (j - c0);
       const BoxDir D = box_dir(dsh);
       for (int i = 0; i < n; ++i) {
-        if (__all((blocked >> i) & 1ull)) continue;
+        if (__all(out || ((blocked >> i) & 1ull))) continue;
         if (hit_box_dir(lane_value(myp, i), D, b) < 1e10f) blocked |= 1ull << i;
       }
     } else {
       for (int i = 0; i < n; ++i) {
-        if (__all((blocked >> i) & 1ull)) continue;
+        if (__all(out || ((blocked >> i) & 1ull))) continue;
         if (hit_obstruction(ob, j, lane_value(myp, i), dsh) < 1e10f) blocked |= 1ull << i;
       }
     }
@@ -609,7 +704,8 @@ IACT_DEV uint64_t far_run_shadow(const TraceArgs& A, const float* rec0, int n, i
 // the other chunk's rays is an obstruction like any other: a ray it blocks
 // is blocked, so the bits are those of two far_run_shadow calls.
 IACT_DEV void far_run_shadow2(const TraceArgs& A, const float* rec0, int n, const int* list,
-                              const uint64_t* bits, const f3* dsh, uint64_t* blk) {
+                              const uint64_t* bits, const f3* dsh, f3 gc, float grho,
+                              uint64_t* blk) {
   const iact_obstructions& ob = A.ob;
   const int lane = lane_id();
   const float* pr = rec0 + (size_t)IACT_REC_WORDS * (lane < n ? lane : 0) + 13;
@@ -626,11 +722,15 @@ IACT_DEV void far_run_shadow2(const TraceArgs& A, const float* rec0, int n, cons
       const int j = __builtin_amdgcn_readlane(list[k0], pos);
       // (a candidate of both chunks is walked with the first)
       if (k0 == 1 && __ballot(in0 && list[0] == j) != 0ull) continue;
+      // directions the candidate cannot reach count as done (far_run_shadow)
+      const uint64_t o0 = obstruction_reaches(ob, j, gc, grho, dsh[0], 0.f) ? 0ull : ~0ull;
+      const uint64_t o1 = obstruction_reaches(ob, j, gc, grho, dsh[1], 0.f) ? 0ull : ~0ull;
+      if (__all((o0 & o1) != 0ull)) continue;
       if (j < c0) {
         const cptr<float> c = as_const(ob.cylinders) + 8 * j;
         const CylDir D0 = cyl_dir(dsh[0], c), D1 = cyl_dir(dsh[1], c);
         for (int i = 0; i < n; ++i) {
-          if (__all(((b0 >> i) & (b1 >> i) & 1ull) != 0ull)) continue;
+          if (__all((((b0 | o0) >> i) & ((b1 | o1) >> i) & 1ull) != 0ull)) continue;
           float t0, t1;
           hit_cylinder_dir2(lane_value(myp, i), D0, D1, c, t0, t1);
           if (t0 < 1e10f) b0 |= 1ull << i;
@@ -640,14 +740,14 @@ IACT_DEV void far_run_shadow2(const TraceArgs& A, const float* rec0, int n, cons
         const cptr<float> b = as_const(ob.boxes) + 6 * (j - c0);
         const BoxDir D0 = box_dir(dsh[0]), D1 = box_dir(dsh[1]);
         for (int i = 0; i < n; ++i) {
-          if (__all(((b0 >> i) & (b1 >> i) & 1ull) != 0ull)) continue;
+          if (__all((((b0 | o0) >> i) & ((b1 | o1) >> i) & 1ull) != 0ull)) continue;
           const f3 o = lane_value(myp, i);
           if (hit_box_dir(o, D0, b) < 1e10f) b0 |= 1ull << i;
           if (hit_box_dir(o, D1, b) < 1e10f) b1 |= 1ull << i;
         }
       } else {
         for (int i = 0; i < n; ++i) {
-          if (__all(((b0 >> i) & (b1 >> i) & 1ull) != 0ull)) continue;
+          if (__all((((b0 | o0) >> i) & ((b1 | o1) >> i) & 1ull) != 0ull)) continue;
           const f3 o = lane_value(myp, i);
           if (hit_obstruction(ob, j, o, dsh[0]) < 1e10f) b0 |= 1ull << i;
           if (hit_obstruction(ob, j, o, dsh[1]) < 1e10f) b1 |= 1ull << i;
