@@ -692,17 +692,16 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
     const uint32_t g = fdiv((uint32_t)item, A.d_sc);
     const int c = (int)((uint32_t)item - g * A.n_sc);
     // lanes past the last source trace its rays with value 0 (their runs
-    // are never added), which keeps every lane of the trace active
-    double sx[SPL], sy[SPL], sz[SPL];
+    // are never added), which keeps every lane of the trace active; the
+    // source's position is read again at each facet run (cached), where it
+    // is used, rather than held across the item's loops
+    int sl[SPL];
     float value[SPL];
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
       const int s = (c * SPL + k) * 64 + lane;
-      const int sl = s < A.ns ? s : A.ns - 1;
-      sx[k] = A.sources[3 * sl];
-      sy[k] = A.sources[3 * sl + 1];
-      sz[k] = A.sources[3 * sl + 2];
-      value[k] = s < A.ns ? A.values[sl] : 0.f;
+      sl[k] = s < A.ns ? s : A.ns - 1;
+      value[k] = s < A.ns ? A.values[sl[k]] : 0.f;
     }
     // sample indices fit in 31 bits (checked on the host): scalar compares
     const int j0 = (int)g * A.sample_group;
@@ -817,7 +816,8 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
       int near = 0;
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
-        e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+        const double* sp = A.sources + 3 * (size_t)sl[k];
+        e[k] = source_offset<SRC>(sp[0], sp[1], sp[2], pos);
         qs[k] = source_qs(A, e[k]);
         near |= qs[k] != 0.f;
         run_s[k] = frsq(dot(e[k], e[k]));  // trace_ray's 1/|u| of a far ray
@@ -953,17 +953,14 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
        item += (long long)grid_blocks() * wpb) {
     const uint32_t g = fdiv((uint32_t)item, A.d_sc);
     const int c = (int)((uint32_t)item - g * A.n_sc);
-    double sx[SPL], sy[SPL], sz[SPL];
+    int sl[SPL];  // the source, read where it is used (k_trace_sm)
     float value[SPL], gval[SPL];
     f3 gsrc[SPL];
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
       const int s = (c * SPL + k) * 64 + lane;
-      const int sl = s < A.ns ? s : A.ns - 1;
-      sx[k] = A.sources[3 * sl];
-      sy[k] = A.sources[3 * sl + 1];
-      sz[k] = A.sources[3 * sl + 2];
-      value[k] = s < A.ns ? A.values[sl] : 0.f;
+      sl[k] = s < A.ns ? s : A.ns - 1;
+      value[k] = s < A.ns ? A.values[sl[k]] : 0.f;
       gval[k] = 0.f;
       gsrc[k] = mk3(0.f, 0.f, 0.f);
     }
@@ -1112,7 +1109,8 @@ __attribute__((amdgpu_waves_per_eu(6))) void k_adj_sm(AdjArgs B) {
       int near = 0;
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
-        e[k] = source_offset<SRC>(sx[k], sy[k], sz[k], pos);
+        const double* sp = A.sources + 3 * (size_t)sl[k];
+        e[k] = source_offset<SRC>(sp[0], sp[1], sp[2], pos);
         qs[k] = source_qs(A, e[k]);
         near |= qs[k] != 0.f;
         // u = e: trace_ray's 1/|u| and d of every far sample
