@@ -858,14 +858,16 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
             const f3 u = e[k];
             dsh[k] = -(frsq(dot(u, u)) * u);  // trace_ray's -d of a far ray
           }
-          // (one chunk at a time: the two-direction walk of the adjoint
-          // costs this kernel two waves per SIMD of occupancy)
           const float* rec0 = (const float*)(recs + 4 * (size_t)jb);
           const cptr<float> gb = as_const(A.gbounds) + 4 * (size_t)(jb >> 6);  // the run's ball
           const f3 gc = mk3(gb[0], gb[1], gb[2]);
+          if constexpr (SPL == 2) {
+            far_run_shadow2(A, rec0, jend - jb, rlist, rbits, dsh, gc, gb[3], blk);
+          } else {
 #pragma unroll
-          for (int k = 0; k < SPL; ++k)
-            blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k], gc, gb[3]);
+            for (int k = 0; k < SPL; ++k)
+              blk[k] = far_run_shadow(A, rec0, jend - jb, rlist[k], rbits[k], dsh[k], gc, gb[3]);
+          }
           if (lean)
             run(F_{}, F_{}, T_{}, T_{});
           else
