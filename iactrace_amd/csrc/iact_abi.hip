@@ -344,8 +344,11 @@ int shape_trace(const iact_scene& sc, int ns, int mode, const iact_launch* L, Tr
   if (mode == IACT_MODE_IMAGE || mode == IACT_MODE_RESPONSE) {
     const size_t per = (size_t)P * sizeof(float);
     const size_t copies = (mode == IACT_MODE_IMAGE && A.det) ? wpb : 1;
-    if (per * copies + tab <= budget) {
-      img = per * copies;
+    // (+ one spare float: the source-major trace's entry for pixel -1; that
+    // trace is never deterministic)
+    const size_t spare = A.det ? 0 : sizeof(float);
+    if (per * copies + spare + tab <= budget) {
+      img = per * copies + spare;
       A.lds_image = 1;
     } else if (A.det) {
       return set_error(IACT_ERR_UNSUPPORTED,

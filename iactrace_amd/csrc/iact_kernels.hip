@@ -664,21 +664,23 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wpb = block_waves();
   const int lane = lane_id();
   const int P = A.S.n_pixels;
-  float* img = smem;
-  int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? P : 0));
+  // the LDS image behind one spare entry, img[-1]: a run of pixel -1 (no
+  // pixel yet, or dropped rays) is added there, without a test
+  float* img = smem + 1;
+  int32_t* tab_lds = (int32_t*)(smem + (A.lds_image ? P + 1 : 0));
   if (A.lds_image)
     for (int i = threadIdx.x; i < P; i += blockDim.x) img[i] = 0.f;
   const int32_t* table =
       KIND == IACT_SENSOR_HEX && A.lds_table ? stage_hex_table(A.S, tab_lds) : tab_lds;
   __syncthreads();
-  // pixel adds go to the LDS image or, when it does not fit, to global
-  // memory -- two separate pointers, so each add is a plain LDS or global
-  // atomic (a pointer that may be either would make it a flat atomic, which
-  // also stalls the record prefetch)
-  auto add_px = [&](int pix, float v) {
+  // a run's add goes to the LDS image or, when it does not fit, to global
+  // memory (pixel -1 and zero runs skipped) -- two separate pointers, so each
+  // add is a plain LDS or global atomic (a pointer that may be either would
+  // make it a flat atomic, which also stalls the record prefetch)
+  auto add_run = [&](int pix, float v) {
     if (A.lds_image)
       lds_add(img + pix, v);
-    else
+    else if (pix >= 0 && v != 0.f)
       unsafeAtomicAdd(A.out64 + pix, (double)v);
   };
   const long long total = (long long)A.nf * A.M;
@@ -768,8 +770,7 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
           // when it ends
           const int pix = grid_pixel<KIND, GEN>(A.S, table, gu, gv, ok);
           if (pix != run_pix[k]) {
-            const float rv = value[k] * fmaf(run_s[k], run_l[k], run_v[k]);
-            if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
+            add_run(run_pix[k], value[k] * fmaf(run_s[k], run_l[k], run_v[k]));
             run_pix[k] = pix;
             run_v[k] = 0.f;
             run_l[k] = 0.f;
@@ -786,7 +787,7 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
         } else {
           splat_taps<KIND>(A.S, table, gu, gv, ok, [&](int pix, float w) {
             const float vw = v * w;
-            if (pix >= 0 && vw != 0.f) add_px(pix, vw);
+            if (pix >= 0 && vw != 0.f) add_run(pix, vw);
           });
         }
       }
@@ -890,8 +891,7 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
     if (!SPLAT) {
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
-        const float rv = value[k] * fmaf(run_s[k], run_l[k], run_v[k]);
-        if (run_pix[k] >= 0 && rv != 0.f) add_px(run_pix[k], rv);
+        add_run(run_pix[k], value[k] * fmaf(run_s[k], run_l[k], run_v[k]));
       }
     }
   }
