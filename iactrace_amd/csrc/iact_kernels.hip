@@ -1485,35 +1485,44 @@ hipError_t launch_chunk_masks(const TraceArgs& A, int source_type, long long n_s
 // One thread per block of 64 consecutive records: the centre of the block's
 // bounding box of sample positions and the largest distance from it.
 __global__ void k_group_bounds(const float* recs, long long total, float* bounds) {
+  // one wave per block of 64 records, lane = record: the extremes and the
+  // largest distance from the centre are wave reductions (min and max are
+  // exact, so the ball is that of a serial pass)
   const long long nb = (total + 63) / 64;
-  for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nb;
-       b += (long long)gridDim.x * blockDim.x) {
-    const long long i0 = 64 * b, i1 = min(i0 + 64, total);
-    f3 lo = mk3(3e38f, 3e38f, 3e38f), hi = mk3(-3e38f, -3e38f, -3e38f);
-    for (long long i = i0; i < i1; ++i) {
-      const float* p = recs + IACT_REC_WORDS * i + 13;
-      lo = mk3(fminf(lo.x, p[0]), fminf(lo.y, p[1]), fminf(lo.z, p[2]));
-      hi = mk3(fmaxf(hi.x, p[0]), fmaxf(hi.y, p[1]), fmaxf(hi.z, p[2]));
+  const int lane = lane_id(), wpb = block_waves();
+  const long long wave = (long long)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (long long b = wave; b < nb; b += (long long)gridDim.x * wpb) {
+    const long long i = 64 * b + lane;
+    const bool in = i < total;  // (a lane past the end repeats the block's first record)
+    const float* p = recs + IACT_REC_WORDS * (in ? i : 64 * b) + 13;
+    const f3 q = mk3(p[0], p[1], p[2]);
+    f3 lo = q, hi = q;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      lo = mk3(fminf(lo.x, __shfl_xor(lo.x, off, 64)), fminf(lo.y, __shfl_xor(lo.y, off, 64)),
+               fminf(lo.z, __shfl_xor(lo.z, off, 64)));
+      hi = mk3(fmaxf(hi.x, __shfl_xor(hi.x, off, 64)), fmaxf(hi.y, __shfl_xor(hi.y, off, 64)),
+               fmaxf(hi.z, __shfl_xor(hi.z, off, 64)));
     }
     const f3 c = 0.5f * (lo + hi);
-    float r2 = 0.f;
-    for (long long i = i0; i < i1; ++i) {
-      const float* p = recs + IACT_REC_WORDS * i + 13;
-      const f3 dv = mk3(p[0], p[1], p[2]) - c;
-      r2 = fmaxf(r2, dot(dv, dv));
-    }
+    const f3 dv = q - c;
+    float r2 = dot(dv, dv);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, off, 64));
     // (1e-6 relative + 1 um of slack for the rounding of centre and radius)
-    float* o = bounds + 4 * b;
-    o[0] = c.x;
-    o[1] = c.y;
-    o[2] = c.z;
-    o[3] = fmaf(sqrtf(r2), 1.000001f, 1e-6f);
+    if (lane == 0) {
+      float* o = bounds + 4 * b;
+      o[0] = c.x;
+      o[1] = c.y;
+      o[2] = c.z;
+      o[3] = fmaf(sqrtf(r2), 1.000001f, 1e-6f);
+    }
   }
 }
 
 hipError_t launch_group_bounds(const TraceArgs& A, float* bounds, hipStream_t st) {
   const long long total = (long long)A.nf * A.M;
-  hipLaunchKernelGGL(k_group_bounds, dim3(grid_for((total + 63) / 64, 256, 4096)), dim3(256), 0,
+  hipLaunchKernelGGL(k_group_bounds, dim3(grid_for((total + 63) / 64, 4, 4096)), dim3(256), 0,
                      st, A.recs, total, bounds);
   return hipGetLastError();
 }
