@@ -137,6 +137,17 @@ __global__ void k_facet_world(iact_facets F, DevSensor S, float* out) {
 }
 
 // ------------------------------------------------------------------ culling
+// Unit direction from a facet centre c towards point source sp and the
+// source's distance, for the cone bounds: the difference in fp64 (sources
+// may be 1e10 m away), its length by one fp64 square root, the rest in fp32
+// (the cones carry 2e-4 rad of slack, far above fp32 rounding).
+IACT_DEV f3 cone_source_dir(const double* sp, f3 c, double& l) {
+  const double ux = sp[0] - (double)c.x, uy = sp[1] - (double)c.y, uz = sp[2] - (double)c.z;
+  l = sqrt(ux * ux + uy * uy + uz * uz);
+  const float il = frcp((float)l);
+  return mk3((float)ux * il, (float)uy * il, (float)uz * il);
+}
+
 // Per facet, the bundle of primary-leg shadow rays: every ray starts inside
 // the ball of radius rho around the facet position (rho = the largest sample
 // distance in the facet frame) and points inside the cone of half angle
@@ -159,10 +170,8 @@ __global__ __launch_bounds__(256) void k_facet_cones(TraceArgs A, int source_typ
     for (int s = t; s < A.ns; s += blockDim.x) {
       f3 u;
       if (source_type == IACT_SOURCE_POINT) {
-        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
-        const double uz = A.sources[3 * s + 2] - (double)c.z;
-        const double il = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
-        u = mk3((float)(ux * il), (float)(uy * il), (float)(uz * il));
+        double l;
+        u = cone_source_dir(A.sources + 3 * (size_t)s, c, l);
       } else {
         u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
                 -(float)A.sources[3 * s + 2]);
@@ -195,11 +204,9 @@ __global__ __launch_bounds__(256) void k_facet_cones(TraceArgs A, int source_typ
       float par = 0.f;
       f3 u;
       if (source_type == IACT_SOURCE_POINT) {
-        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
-        const double uz = A.sources[3 * s + 2] - (double)c.z;
-        const double l = sqrt(ux * ux + uy * uy + uz * uz);
-        u = mk3((float)(ux / l), (float)(uy / l), (float)(uz / l));
-        par = l > (double)rho_f ? asinf((float)((double)rho_f / l)) : 3.1415927f;
+        double l;
+        u = cone_source_dir(A.sources + 3 * (size_t)s, c, l);
+        par = l > (double)rho_f ? asinf(rho_f * frcp((float)l)) : 3.1415927f;
       } else {
         u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
                 -(float)A.sources[3 * s + 2]);
@@ -278,10 +285,8 @@ __global__ void k_chunk_masks(TraceArgs A, int source_type, long long n_sources,
     for (long long s = s0; s < s1; ++s) {
       f3 u;
       if (source_type == IACT_SOURCE_POINT) {
-        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
-        const double uz = A.sources[3 * s + 2] - (double)c.z;
-        const double il = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
-        u = mk3((float)(ux * il), (float)(uy * il), (float)(uz * il));
+        double l;
+        u = cone_source_dir(A.sources + 3 * (size_t)s, c, l);
       } else {
         u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
                 -(float)A.sources[3 * s + 2]);
@@ -296,11 +301,9 @@ __global__ void k_chunk_masks(TraceArgs A, int source_type, long long n_sources,
       float par = 0.f;
       f3 u;
       if (source_type == IACT_SOURCE_POINT) {
-        const double ux = A.sources[3 * s] - (double)c.x, uy = A.sources[3 * s + 1] - (double)c.y;
-        const double uz = A.sources[3 * s + 2] - (double)c.z;
-        const double l = sqrt(ux * ux + uy * uy + uz * uz);
-        u = mk3((float)(ux / l), (float)(uy / l), (float)(uz / l));
-        par = l > (double)rho ? asinf((float)((double)rho / l)) : 3.1415927f;
+        double l;
+        u = cone_source_dir(A.sources + 3 * (size_t)s, c, l);
+        par = l > (double)rho ? asinf(rho * frcp((float)l)) : 3.1415927f;
       } else {
         u = mk3(-(float)A.sources[3 * s], -(float)A.sources[3 * s + 1],
                 -(float)A.sources[3 * s + 2]);
