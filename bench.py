@@ -171,40 +171,40 @@ def c5(ia, configs, n_instances, steps, warmup, want_parity, want_plain):
     bwd_tf = n * 2 * f32 / (b_ms * 1e-3) / 1e12
     fwd_tf = n * f32 / (f_ms * 1e-3) / 1e12
     it_tf = n * 3 * f32 / it_s / 1e12
+    flops = {"forward": f32, "adjoint": 2 * f32,
+             "convention": "fwd+bwd = 3 x fwd; FP32 only; no credit for obstruction tests; "
+                           "the 12 FP64 flops of a point source's direction not counted"}
+    peak_note = ("157.3 TFLOP/s assumes packed FP32 (two FMAs per lane per instruction); these "
+                 "kernels are built without packed FP32.  frac counts algorithmic flops, which "
+                 "include work the kernels do once per facet and source instead of per ray, "
+                 "and the adjoint at twice the forward's flops (so its frac can exceed 1); "
+                 "valu_issue (rocprofv3 counters: VALU instructions x 4 cycles / (SIMDs x "
+                 "cycles)) is the hardware's measured utilisation")
+
+    def roof(tf, ms, name, what, counters, key):
+        return {"bound": "valu", "achieved": tf, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                "frac": tf * 1e12 / PEAK_FP32,
+                "traffic": profile_value(counters, "hbm_bytes_per_launch"),
+                "kernel": f"{name} ({what}, one launch per instance), HIP events around the "
+                          "batched call on the launch stream, mean of 3",
+                "kernel_ms": ms, "rocprof_kernel_ms": profile_value(counters, key),
+                "fp32_flops_per_ray": flops, "peak_note": peak_note,
+                "valu_issue": profile_value(counters, "valu_issue_frac")}
+
+    fwd_roof = roof(fwd_tf, f_ms, "k_trace_sm", "source-major forward",
+                    "k_trace_sm_C5_counters.txt", "kernel_ms_per_forward")
+    adj_roof = roof(bwd_tf, b_ms, "k_adj_sm", "source-major adjoint",
+                    "k_adj_sm_C5_counters.txt", "kernel_ms_per_backward")
     out = {
         "value": n / it_s, "ms_per_step": 1e3 * it_s, "rays_per_step": n,
         "event_ms_per_step": float(np.mean(ev)),
         "loss": {"first_step": loss_curve[0], "last_step": loss_curve[-1],
                  "steps": len(loss_curve), "falls": loss_curve[-1] < loss_curve[0]},
-        "roofline": {
-            "bound": "valu", "achieved": bwd_tf, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-            "frac": bwd_tf * 1e12 / PEAK_FP32,
-            "traffic": profile_value("k_adj_sm_C5_counters.txt", "hbm_bytes_per_launch"),
-            "kernel": "k_adj_sm (source-major adjoint, one launch per instance), HIP "
-                      "events around iact_trace_batched_bwd on the launch stream, mean of 3",
-            "kernel_ms": b_ms,
-            "rocprof_kernel_ms": profile_value("k_adj_sm_C5_counters.txt",
-                                               "kernel_ms_per_backward"),
-            "fp32_flops_per_ray": {"forward": f32, "adjoint": 2 * f32,
-                                   "convention": "fwd+bwd = 3 x fwd; FP32 only; no credit for "
-                                                 "obstruction tests; the 12 FP64 flops of a "
-                                                 "point source's direction not counted"},
-            "peak_note": "157.3 TFLOP/s assumes packed FP32 (two FMAs per lane per "
-                         "instruction); these kernels are built without packed FP32.  frac "
-                         "counts algorithmic flops, which include work the kernels do once "
-                         "per facet and source instead of per ray, and the adjoint at twice "
-                         "the forward's flops; valu_issue (rocprofv3 counters: VALU "
-                         "instructions x 4 cycles / (SIMDs x cycles)) is the hardware's "
-                         "measured utilisation",
-            "valu_issue": profile_value("k_adj_sm_C5_counters.txt", "valu_issue_frac"),
-        },
-        "forward_kernel": {"kernel": "batched forward (k_trace_sm per instance)",
-                           "ms": f_ms, "achieved_tflops": fwd_tf,
-                           "frac": fwd_tf * 1e12 / PEAK_FP32,
-                           "rocprof_kernel_ms": profile_value("k_trace_sm_C5_counters.txt",
-                                                              "kernel_ms_per_forward"),
-                           "valu_issue": profile_value("k_trace_sm_C5_counters.txt",
-                                                       "valu_issue_frac")},
+        "forward_kernel": fwd_roof,
+        "adjoint_kernel": adj_roof,
+        # the dominant kernel: the one of the two per-pass kernels that took
+        # longer in this run (HIP events)
+        "roofline": dict(fwd_roof if f_ms >= b_ms else adj_roof),
         "iteration_fp32_frac": it_tf * 1e12 / PEAK_FP32,
     }
     if want_plain:
@@ -511,6 +511,7 @@ def main():
                    "mode": "batched forward + backward (alignment-fit iteration)"},
         "roofline": r["roofline"],
         "forward_kernel": r["forward_kernel"],
+        "adjoint_kernel": r["adjoint_kernel"],
         "iteration_fp32_frac": r["iteration_fp32_frac"],
         "loss": r["loss"],
     }
