@@ -654,12 +654,13 @@ IACT_DEV bool refine_candidates(const TraceArgs& A, int f, int c, int blk, int l
 // item takes one of two paths, uniform over the wave: runs without an
 // obstruction candidate for any of the lane's chunks (most of them when
 // culling) are traced without shadow tests, runs with one take the shadow
-// path (refined candidates, far_run_shadow bits or per-ray tests).  The
-// kernel is held to 80 VGPRs (6 waves per SIMD; the shadow path alone would
-// take 84): on the alignment fit's instances one such launch is 7.6 % faster
+// path (refined candidates, far_run_shadow2 bits or per-ray tests).  The
+// kernel is held to 80 VGPRs (6 waves per SIMD; unhinted it takes 94, 5
+// waves): on the alignment fit's instances one such launch was 7.6 % faster
 // than the two launches of an earlier split by path (each over every item,
-// each with its own register allocation), and 7 % faster than at 5 or 7
-// waves, whose register limits move spills into the per-ray loop.
+// each with its own register allocation), and it is 5 % faster than at 5
+// waves and 8 % faster than at 7, whose register limit moves spills into the
+// per-ray loop.
 template <int SRC, int KIND, int SPLAT, bool STAGES, bool AXIS, int OBST, int SPL>
 __global__ __launch_bounds__(IACT_MAX_BLOCK)
 __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) {
@@ -927,10 +928,10 @@ __attribute__((amdgpu_waves_per_eu(OBST ? 6 : 1))) void k_trace_sm(TraceArgs A) 
 // fp64 per-facet sums go to k_torque_to_euler, which dots them with the axes.
 // Source and value gradients are summed per lane (per source) over an item's
 // samples.
-// Held to 80 VGPRs (6 waves per SIMD; the shadow path alone would take 112):
-// on the alignment fit's instances 14 % faster than the two launches of an
-// earlier split by path at their natural allocations, 1 % faster than at 5
-// waves.
+// Held to 80 VGPRs (6 waves per SIMD; unhinted the rotation-only kernel
+// takes 92, the other 104): on the alignment fit's instances 14 % faster than
+// the two launches of an earlier split by path at their natural allocations,
+// and 3 % faster than the rotation-only kernel at 5 waves without spills.
 // GX false: the launch wants rotation gradients only (no value, source or
 // position gradients), and the kernel has no code or registers for the rest.
 template <int SRC, int KIND, bool AXIS, int OBST, int SPL, bool GX>
